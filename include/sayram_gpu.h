@@ -1,0 +1,164 @@
+/*
+ * sayram_gpu.h -- C ABI of the MI355X (gfx950) implicit PPFV time step.
+ *
+ * Drop-in for the reference's Solver (source/Solver.h:18-82). The reference has
+ * no FFI: its only boundary is the C++ class Solver, constructed in
+ * source/main.cc:51, stepped in source/main.cc:80 (Solver::update,
+ * source/Solver.cc:270-290) and read in source/main.cc:74,83 (Solver::f(),
+ * source/Solver.h:24). The C++ shim GpuSolver (sayram-2d_amd/host/GpuSolver.h)
+ * reproduces that class surface on top of the functions below.
+ *
+ * Conventions
+ *  - Every function returns SG_OK (0) or a negative sg_status; sg_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - Cell arrays crossing this boundary use the reference's Xtensor2d layout:
+ *    shape (nx, ny), element (i, j) at i*ny + j (source/common.h:28).
+ *  - Boundary sides are ordered XMIN, XMAX, YMIN, YMAX (source/BCTypes.h:13).
+ *    Dirichlet vertex values for XMIN/XMAX hold ny+1 values (vertex j), for
+ *    YMIN/YMAX nx+1 values (vertex i), as Equation::dirichlet_vertex_value
+ *    supplies them (source/Equation.h:59-65, source/Solver.cc:385-422).
+ *  - The library owns all device memory; host buffers are copied on call.
+ *  - A handle is not thread-safe: use one handle per host thread.
+ */
+#ifndef SAYRAM_GPU_H
+#define SAYRAM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_ABI_VERSION 1
+
+typedef enum {
+  SG_OK = 0,
+  SG_ERR_ARG = -1,       /* invalid argument (shape, null pointer, bad enum) */
+  SG_ERR_HIP = -2,       /* HIP runtime failure, or no GPU */
+  SG_ERR_NOMEM = -3,     /* device allocation failed or HBM budget exceeded */
+  SG_ERR_NOCONV = -4,    /* Krylov solve did not reach its tolerance */
+  SG_ERR_BC = -5,        /* missing Dirichlet values (source/Solver.cc:392-394) */
+  SG_ERR_STATE = -6      /* call not valid in the handle's current state */
+} sg_status;
+
+typedef enum { SG_DIRICHLET = 0, SG_ZEROFLUX = 1 } sg_bc_type;   /* source/BCTypes.h:16 */
+typedef enum { SG_XMIN = 0, SG_XMAX = 1, SG_YMIN = 2, SG_YMAX = 3 } sg_side;
+typedef enum { SG_FP64 = 0, SG_MIXED = 1 } sg_precision;
+
+/* Grid2D (source/Grid2D.h:16-68): strictly increasing cell edges. */
+typedef struct {
+  int nx, ny;
+  const double *x_edges;   /* nx+1 */
+  const double *y_edges;   /* ny+1 */
+} sg_grid;
+
+/* Equation's per-cell arrays (source/Equation.h:44-50, 67-74), (nx,ny) layout.
+ * inv_tau may be NULL (1/tau = 0, source/Equation.h:39-40). */
+typedef struct {
+  const double *G, *Dxx, *Dxy, *Dyy, *inv_tau;
+} sg_coeffs;
+
+/* Boundary types and Dirichlet vertex values; values[s] may be NULL for a
+ * zero-flux side. */
+typedef struct {
+  int type[4];
+  const double *values[4];
+} sg_bc;
+
+/* Solver options. Zero-initialised options select the defaults. */
+typedef struct {
+  int precision;        /* sg_precision; default SG_FP64 */
+  double rtol;          /* stop when ||R - M f||_2 <= rtol ||R||_2; default 1e-14 */
+  int max_iters;        /* BiCGSTAB iteration cap per step (FP64) or refinement
+                           sweeps (MIXED); default 200 / 12 */
+  double inner_rtol;    /* MIXED: FP32 inner solve tolerance; default 1e-5 */
+  int max_inner;        /* MIXED: FP32 inner iteration cap; default 40 */
+  int check_every;      /* iterations between host convergence checks; default 2 */
+  int smoother;         /* bit0: pre x-lines, bit1: pre y-lines, bit2: post x,
+                           bit3: post y; default 15 (all) */
+  int use_graph;        /* capture iteration chunks into hipGraphs; default 1 */
+  int reserved[6];
+} sg_opts;
+
+typedef struct {
+  int64_t steps;              /* steps taken since creation / last set_f */
+  int last_iters;             /* Krylov iterations of the last step (sum of
+                                 inner iterations for MIXED) */
+  int last_refinements;       /* MIXED: FP64 refinement sweeps of the last step */
+  double last_rel_residual;   /* ||R - M f||/||R|| before clamping, last step */
+  double min_f;               /* min f over the grid after the last step (>= 0) */
+  int64_t clamp_count;        /* cumulative count of cells clamped to 0 */
+  int64_t total_iters;        /* cumulative Krylov iterations */
+  int status;                 /* sg_status of the last step */
+} sg_step_stats;
+
+typedef struct sg_solver sg_solver;   /* one instance (Solver equivalent) */
+typedef struct sg_batch sg_batch;     /* many independent instances */
+
+/* --- library ------------------------------------------------------------ */
+int sg_abi_version(void);
+const char *sg_last_error(void);
+/* Number of visible GPUs (0 when none); never fails. */
+int sg_device_count(void);
+
+/* --- single instance: Solver surface ------------------------------------ */
+/* Solver::Solver + Solver::init (source/Solver.cc:14-55): uploads the grid,
+ * G and D (Lambda = G*D, source/Solver.cc:57-65), 1/tau, boundary data and f0
+ * (Equation::init_f values). dt as Mesh::dt (source/Mesh.h:55). */
+int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, const double *f0,
+              double dt, const sg_opts *opts, sg_solver **out);
+/* Dirichlet vertex values of one side at the next step's time t
+ * (fill_vertex_from_bcs, source/Solver.cc:385-422). */
+int sg_set_bc_values(sg_solver *h, int side, const double *values);
+/* Re-upload G and D after Equation::update (source/Solver.cc:287-288). */
+int sg_set_coeffs(sg_solver *h, const sg_coeffs *coeffs);
+/* nsteps x Solver::update (source/Solver.cc:270-290); synchronous. */
+int sg_step(sg_solver *h, int nsteps);
+/* Solver::f() (source/Solver.h:24): f in (nx,ny) layout. */
+int sg_get_f(sg_solver *h, double *out);
+/* Resume: replace f and the step counter (the whole solver state). */
+int sg_set_f(sg_solver *h, const double *f, int64_t istep);
+/* Solver::t() = istep*dt (source/Solver.h:23). */
+double sg_time(const sg_solver *h);
+int sg_stats(const sg_solver *h, sg_step_stats *out);
+void sg_destroy(sg_solver *h);
+
+/* --- batched: many instances of different sizes per step ---------------- */
+/* hbm_budget_bytes = 0 means no budget beyond the device's free memory. */
+int sg_batch_create(const sg_opts *opts, size_t hbm_budget_bytes, sg_batch **out);
+/* Device bytes one instance of this shape would occupy (admission control). */
+size_t sg_instance_bytes(int nx, int ny, int precision);
+/* Adds an instance; returns its slot id in *slot. SG_ERR_NOMEM when the
+ * budget would be exceeded (nothing is allocated then). */
+int sg_batch_add(sg_batch *b, const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc,
+                 const double *f0, double dt, int *slot);
+int sg_batch_remove(sg_batch *b, int slot);
+/* One implicit step for every resident instance, nsteps times. */
+int sg_batch_step(sg_batch *b, int nsteps);
+/* As sg_batch_step, but instance k only advances while its step count is
+ * below target_steps[k] (indexed by slot; entries of free slots ignored). */
+int sg_batch_step_until(sg_batch *b, const int64_t *target_steps, int nslots, int nsteps);
+int sg_batch_get_f(sg_batch *b, int slot, double *out);
+int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep);
+int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values);
+int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out);
+int sg_batch_size(const sg_batch *b);
+size_t sg_batch_bytes_in_use(const sg_batch *b);
+void sg_batch_destroy(sg_batch *b);
+
+/* --- instrumentation (benchmarks and tests) ----------------------------- */
+/* Assembly only: writes the 5 FP64 bands (d|w|e|s|n, K = j*nx+i order) and R
+ * of the next step into host buffers, for parity checks of the assembly. */
+int sg_debug_assemble(sg_solver *h, double *bands5, double *R);
+/* Kernel timing with HIP events on the library's stream: kernel_id selects the
+ * kernel to bracket (see sg_kernel_name), -1 disables. */
+int sg_batch_time_kernel(sg_batch *b, int kernel_id);
+int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes);
+const char *sg_kernel_name(int kernel_id);
+int sg_solver_batch(sg_solver *h, sg_batch **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

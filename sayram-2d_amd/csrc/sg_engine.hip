@@ -1,0 +1,898 @@
+// sg_engine.hip -- host orchestration of the gfx950 PPFV step and the C ABI
+// declared in include/sayram_gpu.h.
+//
+// A batch owns one HIP stream, a table of instances (one device slab each) and
+// the per-level block/line maps the batched kernels walk. A single solver
+// (the reference's Solver, source/Solver.h:18-82) is a batch of one.
+#include "sayram_gpu.h"
+#include "sg_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace sg {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(SG_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+static size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Dims { int nx, ny; };
+
+// Coarsen both directions (ceil halving) until one of them is 1; the last
+// level is then solved exactly by one line solve.
+static std::vector<Dims> level_dims(int nx, int ny) {
+  std::vector<Dims> v{{nx, ny}};
+  while (std::min(nx, ny) > 1) {
+    nx = (nx + 1) / 2;
+    ny = (ny + 1) / 2;
+    v.push_back({nx, ny});
+  }
+  return v;
+}
+
+// Assigns every device pointer of an instance inside one slab starting at
+// `base` (0 to size it); returns the slab size in bytes.
+static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    void *p = reinterpret_cast<void *>(base + off);
+    off += a256(bytes);
+    return p;
+  };
+  const size_t N = (size_t)nx * ny;
+  const size_t D = sizeof(double);
+  const size_t T = mixed ? sizeof(float) : sizeof(double);
+  const std::vector<Dims> dims = level_dims(nx, ny);
+  I.nx = nx; I.ny = ny; I.nlev = (int)dims.size();
+  I.nblk0 = (int)((N + kBlock - 1) / kBlock);
+  I.mixed = mixed ? 1 : 0;
+  I.G = (double *)take(N * D); I.Lxx = (double *)take(N * D); I.Lxy = (double *)take(N * D);
+  I.Lyy = (double *)take(N * D); I.itau = (double *)take(N * D);
+  I.f = (double *)take(N * D); I.R = (double *)take(N * D); I.x = (double *)take(N * D);
+  for (int b = 0; b < 5; ++b) I.band64[b] = (double *)take(N * D);
+  I.r64 = mixed ? (double *)take(N * D) : nullptr;
+  I.xe = (double *)take((nx + 1) * D); I.ye = (double *)take((ny + 1) * D);
+  I.wx0 = (double *)take((nx + 1) * D); I.wx1 = (double *)take((nx + 1) * D);
+  I.wy0 = (double *)take((ny + 1) * D); I.wy1 = (double *)take((ny + 1) * D);
+  I.bcv[0] = (double *)take((ny + 1) * D); I.bcv[1] = (double *)take((ny + 1) * D);
+  I.bcv[2] = (double *)take((nx + 1) * D); I.bcv[3] = (double *)take((nx + 1) * D);
+  I.partial = (double *)take((size_t)4 * I.nblk0 * D);
+  for (int v = 0; v < kNumVec; ++v) I.kv[v] = (v == VXK && !mixed) ? nullptr : take(N * T);
+  for (int l = 0; l < kMaxLev; ++l) {
+    Lev &L = I.lev[l];
+    std::memset(&L, 0, sizeof(L));
+    if (l >= I.nlev) continue;
+    L.nx = dims[l].nx; L.ny = dims[l].ny;
+    const size_t Nl = (size_t)L.nx * L.ny;
+    if (l == 0) {
+      for (int b = 0; b < 5; ++b) L.band[b] = mixed ? take(N * T) : (void *)I.band64[b];
+    } else {
+      for (int b = 0; b < 9; ++b) L.band[b] = take(Nl * T);
+      L.x = take(Nl * T);
+      L.b = take(Nl * T);
+    }
+    L.cp = take(Nl * T);
+    L.dp = take(Nl * T);
+  }
+  return off;
+}
+
+struct Slot {
+  bool used = false;
+  int nx = 0, ny = 0;
+  bool mixed = false;
+  void *slab = nullptr;
+  size_t bytes = 0;
+  InstDev dev{};
+  int64_t steps = 0;
+  int64_t target = 0;
+  double dt = 0;
+  sg_step_stats st{};
+};
+
+struct Maps {
+  std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
+  std::vector<LineMap> smap[kMaxLev][2][2], cxmap[kMaxLev], cymap[kMaxLev];
+};
+
+struct DevMaps {   // offsets into one device buffer
+  const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
+  int ncmap[kMaxLev], npmap[kMaxLev];
+  const LineMap *smap[kMaxLev][2][2], *cxmap[kMaxLev], *cymap[kMaxLev];
+  int nsmap[kMaxLev][2][2], ncx[kMaxLev], ncy[kMaxLev];
+  int maxlev;
+};
+
+static const char *kKernelNames[] = {"assemble",   "rap",        "line_l0",    "line_coarse",
+                                     "restrict",   "prolong",    "spmv_v",     "spmv_t",
+                                     "vec_update", "ir_residual", "finalize"};
+constexpr int kNumKernelIds = 11;
+
+struct Batch {
+  sg_opts opts{};
+  size_t budget = 0, in_use = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Slot> slots;
+  int capacity = 0;
+  InstDev *d_insts = nullptr;   // compact table
+  KScal *d_scal = nullptr;      // by slot
+  unsigned *d_ticket = nullptr; // 2 per slot
+  int *d_part = nullptr;        // by compact index
+  int *d_cnt = nullptr;         // [0] active, [1] oactive
+  int *h_cnt = nullptr;         // pinned
+  void *d_maps = nullptr;
+  size_t d_maps_bytes = 0;
+  DevMaps dm{};
+  std::vector<int> order;       // compact index -> slot
+  std::vector<int> part_host;
+  bool dirty = true;
+  double *d_scratch = nullptr;  // layout transposes
+  size_t scratch_elems = 0;
+  std::vector<KScal> h_scal;
+  // kernel timing
+  int time_id = -1;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double t_ms = 0;
+  int64_t t_launches = 0;
+
+  ~Batch();
+  int init(const sg_opts *o, size_t budget);
+  int ensure_capacity(int cap);
+  int add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const double *f0, double dt,
+          int *slot);
+  int upload_coeffs(Slot &s, const sg_coeffs *c);
+  int upload_bc(Slot &s, int side, const double *vals);
+  int remove(int slot);
+  int rebuild();
+  int step(int nsteps, const int64_t *targets, int ntargets);
+  int step_once();
+  template <typename T> int krylov(int maxit, bool xstate);
+  template <typename T> void iteration(bool xstate);
+  template <typename T> void vcycle(int l, int xid, int bid);
+  int read_counter(int which, int *out);
+  int scratch(size_t n);
+  int get_f(int slot, double *out);
+  int set_f(int slot, const double *f, int64_t istep);
+  int debug_assemble(int slot, double *bands, double *R);
+  // timing helpers
+  void tbegin(int id);
+  void tend(int id);
+  int collect_time();
+};
+
+Batch::~Batch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &s : slots)
+    if (s.slab) (void)hipFree(s.slab);
+  (void)hipFree(d_insts); (void)hipFree(d_scal); (void)hipFree(d_ticket); (void)hipFree(d_part); (void)hipFree(d_cnt);
+  (void)hipFree(d_maps); (void)hipFree(d_scratch);
+  if (h_cnt) (void)hipHostFree(h_cnt);
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static void apply_defaults(sg_opts &o) {
+  if (o.rtol <= 0) o.rtol = 1e-14;
+  if (o.max_iters <= 0) o.max_iters = o.precision == SG_MIXED ? 12 : 200;
+  if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
+  if (o.max_inner <= 0) o.max_inner = 40;
+  if (o.check_every <= 0) o.check_every = 2;
+  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 15;
+}
+
+int Batch::init(const sg_opts *o, size_t b) {
+  if (o) opts = *o;
+  if (opts.precision != SG_FP64 && opts.precision != SG_MIXED)
+    return fail(SG_ERR_ARG, "opts.precision must be SG_FP64 or SG_MIXED");
+  apply_defaults(opts);
+  budget = b;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SG_ERR_HIP, "no GPU visible: the PPFV step has no CPU fallback");
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&d_cnt, 2 * sizeof(int)));
+  HIPCHK(hipHostMalloc((void **)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
+  return ensure_capacity(16);
+}
+
+int Batch::ensure_capacity(int cap) {
+  if (cap <= capacity) return SG_OK;
+  int nc = std::max(cap, capacity * 2);
+  KScal *ns = nullptr;
+  unsigned *nt = nullptr;
+  InstDev *ni = nullptr;
+  int *np = nullptr;
+  HIPCHK(hipMalloc(&ns, sizeof(KScal) * nc));
+  HIPCHK(hipMalloc(&nt, sizeof(unsigned) * 2 * nc));
+  HIPCHK(hipMalloc(&ni, sizeof(InstDev) * nc));
+  HIPCHK(hipMalloc(&np, sizeof(int) * nc));
+  HIPCHK(hipMemsetAsync(ns, 0, sizeof(KScal) * nc, stream));
+  HIPCHK(hipMemsetAsync(nt, 0, sizeof(unsigned) * 2 * nc, stream));
+  if (capacity) {
+    HIPCHK(hipMemcpyAsync(ns, d_scal, sizeof(KScal) * capacity, hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(nt, d_ticket, sizeof(unsigned) * 2 * capacity, hipMemcpyDeviceToDevice,
+                          stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  (void)hipFree(d_scal); (void)hipFree(d_ticket); (void)hipFree(d_insts); (void)hipFree(d_part);
+  d_scal = ns; d_ticket = nt; d_insts = ni; d_part = np;
+  capacity = nc;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    slots[k].dev.sc = d_scal + k;
+    slots[k].dev.ticket = d_ticket + 2 * k;
+  }
+  h_scal.resize(nc);
+  dirty = true;
+  return SG_OK;
+}
+
+static int check_grid(const sg_grid *g) {
+  if (!g || !g->x_edges || !g->y_edges) return fail(SG_ERR_ARG, "grid: null pointer");
+  if (g->nx < 1 || g->ny < 1) return fail(SG_ERR_ARG, "grid: nx and ny must be >= 1");
+  if ((long long)g->nx * g->ny > (1LL << 31) - 1) return fail(SG_ERR_ARG, "grid: too many cells");
+  if (level_dims(g->nx, g->ny).size() > (size_t)kMaxLev)
+    return fail(SG_ERR_ARG, "grid: too many multigrid levels");
+  // Grid2D::validate, source/Grid2D.h:44-67
+  for (int i = 0; i < g->nx; ++i)
+    if (!(g->x_edges[i + 1] > g->x_edges[i]))
+      return fail(SG_ERR_ARG, "Grid2D: x_edges must be strictly increasing at i=" + std::to_string(i));
+  for (int j = 0; j < g->ny; ++j)
+    if (!(g->y_edges[j + 1] > g->y_edges[j]))
+      return fail(SG_ERR_ARG, "Grid2D: y_edges must be strictly increasing at j=" + std::to_string(j));
+  return SG_OK;
+}
+
+static int check_coeffs(const sg_coeffs *c) {
+  if (!c || !c->G || !c->Dxx || !c->Dxy || !c->Dyy) return fail(SG_ERR_ARG, "coeffs: null pointer");
+  return SG_OK;
+}
+
+// Interpolation weights of the vertex rule (source/Solver.cc:326-381), evaluated
+// exactly as written there (no fused multiply-add).
+static void vertex_weights(const double *e, int n, std::vector<double> &w0, std::vector<double> &w1) {
+#pragma clang fp contract(off)
+  w0.assign(n + 1, 0.0);
+  w1.assign(n + 1, 0.0);
+  for (int i = 1; i < n; ++i) {
+    const double v = e[i];
+    const double cL = 0.5 * (e[i - 1] + e[i]);
+    const double cR = 0.5 * (e[i] + e[i + 1]);
+    const double den = (cR - cL);
+    w0[i] = (cR - v) / den;
+    w1[i] = (v - cL) / den;
+  }
+}
+
+int Batch::upload_coeffs(Slot &s, const sg_coeffs *c) {
+  const int nx = s.nx, ny = s.ny;
+  const size_t N = (size_t)nx * ny;
+  std::vector<double> h(5 * N);
+  double *G = h.data(), *Lxx = G + N, *Lxy = Lxx + N, *Lyy = Lxy + N, *it = Lyy + N;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const size_t q = (size_t)i * ny + j, K = (size_t)j * nx + i;
+      const double g = c->G[q];
+      if (!std::isfinite(g) || !std::isfinite(c->Dxx[q]) || !std::isfinite(c->Dxy[q]) ||
+          !std::isfinite(c->Dyy[q]))
+        return fail(SG_ERR_ARG, "coeffs: non-finite G or D at cell (" + std::to_string(i) + "," +
+                                    std::to_string(j) + ")");
+      G[K] = g;
+      // Lambda = G * D (source/Solver.cc:61-62)
+      Lxx[K] = c->Dxx[q] * g;
+      Lxy[K] = c->Dxy[q] * g;
+      Lyy[K] = c->Dyy[q] * g;
+      it[K] = c->inv_tau ? c->inv_tau[q] : 0.0;
+    }
+  HIPCHK(hipMemcpyAsync((void *)s.dev.G, G, N * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void *)s.dev.Lxx, Lxx, N * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void *)s.dev.Lxy, Lxy, N * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void *)s.dev.Lyy, Lyy, N * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void *)s.dev.itau, it, N * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SG_OK;
+}
+
+int Batch::upload_bc(Slot &s, int side, const double *vals) {
+  const int n = (side < 2) ? s.ny + 1 : s.nx + 1;
+  if (!vals) return fail(SG_ERR_BC, "Dirichlet BC: missing value.");
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(vals[k])) return fail(SG_ERR_BC, "Dirichlet BC: non-finite value.");
+  HIPCHK(hipMemcpyAsync((void *)s.dev.bcv[side], vals, n * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SG_OK;
+}
+
+int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const double *f0, double dt,
+               int *slot_out) {
+  int rc;
+  if ((rc = check_grid(g))) return rc;
+  if ((rc = check_coeffs(c))) return rc;
+  if (!bc || !f0 || !slot_out) return fail(SG_ERR_ARG, "bc, f0 and slot must not be null");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(SG_ERR_ARG, "dt must be positive");
+  for (int s = 0; s < 4; ++s) {
+    if (bc->type[s] != SG_DIRICHLET && bc->type[s] != SG_ZEROFLUX)
+      return fail(SG_ERR_ARG, "bc type must be SG_DIRICHLET or SG_ZEROFLUX");
+    if (bc->type[s] == SG_DIRICHLET && !bc->values[s])
+      return fail(SG_ERR_BC, "Dirichlet BC: missing value.");
+  }
+  const bool mixed = opts.precision == SG_MIXED;
+  InstDev tmp{};
+  const size_t bytes = carve(0, g->nx, g->ny, mixed, tmp);
+  if (budget && in_use + bytes > budget)
+    return fail(SG_ERR_NOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
+                                  " more bytes, " + std::to_string(budget - in_use) + " free");
+  int slot = -1;
+  for (size_t k = 0; k < slots.size(); ++k)
+    if (!slots[k].used) { slot = (int)k; break; }
+  if (slot < 0) {
+    slot = (int)slots.size();
+    if ((rc = ensure_capacity(slot + 1))) return rc;
+    slots.emplace_back();
+  }
+  Slot &s = slots[slot];
+  void *slab = nullptr;
+  if (hipMalloc(&slab, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  s = Slot{};
+  s.nx = g->nx; s.ny = g->ny; s.mixed = mixed; s.slab = slab; s.bytes = bytes; s.dt = dt;
+  carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, mixed, s.dev);
+  s.dev.dt = dt;
+  s.dev.sc = d_scal + slot;
+  s.dev.ticket = d_ticket + 2 * slot;
+  s.dev.bcmask = 0;
+  for (int k = 0; k < 4; ++k)
+    if (bc->type[k] == SG_DIRICHLET) s.dev.bcmask |= 1 << k;
+  auto undo = [&](int code) {
+    (void)hipFree(s.slab);
+    s = Slot{};
+    return code;
+  };
+  const int nx = g->nx, ny = g->ny;
+  const size_t N = (size_t)nx * ny;
+  if (hipMemcpy((void *)s.dev.xe, g->x_edges, (nx + 1) * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy((void *)s.dev.ye, g->y_edges, (ny + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return undo(fail(SG_ERR_HIP, "upload of the grid failed"));
+  std::vector<double> w0, w1;
+  vertex_weights(g->x_edges, nx, w0, w1);
+  (void)hipMemcpy((void *)s.dev.wx0, w0.data(), (nx + 1) * 8, hipMemcpyHostToDevice);
+  (void)hipMemcpy((void *)s.dev.wx1, w1.data(), (nx + 1) * 8, hipMemcpyHostToDevice);
+  vertex_weights(g->y_edges, ny, w0, w1);
+  (void)hipMemcpy((void *)s.dev.wy0, w0.data(), (ny + 1) * 8, hipMemcpyHostToDevice);
+  (void)hipMemcpy((void *)s.dev.wy1, w1.data(), (ny + 1) * 8, hipMemcpyHostToDevice);
+  if ((rc = upload_coeffs(s, c))) return undo(rc);
+  for (int k = 0; k < 4; ++k)
+    if (bc->type[k] == SG_DIRICHLET && (rc = upload_bc(s, k, bc->values[k]))) return undo(rc);
+  std::vector<double> fk(N);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double v = f0[(size_t)i * ny + j];
+      if (!std::isfinite(v)) return undo(fail(SG_ERR_ARG, "f0: non-finite value"));
+      fk[(size_t)j * nx + i] = v;
+    }
+  if (hipMemcpy(s.dev.f, fk.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return undo(fail(SG_ERR_HIP, "upload of f0 failed"));
+  s.used = true;
+  in_use += bytes;
+  dirty = true;
+  *slot_out = slot;
+  return SG_OK;
+}
+
+int Batch::remove(int slot) {
+  if (slot < 0 || slot >= (int)slots.size() || !slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  HIPCHK(hipStreamSynchronize(stream));
+  (void)hipFree(slots[slot].slab);
+  in_use -= slots[slot].bytes;
+  slots[slot] = Slot{};
+  dirty = true;
+  return SG_OK;
+}
+
+int Batch::rebuild() {
+  Maps M;
+  order.clear();
+  std::vector<InstDev> table;
+  int maxlev = 0;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    if (!slots[k].used) continue;
+    const int c = (int)order.size();
+    order.push_back((int)k);
+    const InstDev &I = slots[k].dev;
+    table.push_back(I);
+    maxlev = std::max(maxlev, I.nlev);
+    for (int l = 0; l < I.nlev; ++l) {
+      const int lx = I.lev[l].nx, ly = I.lev[l].ny, Nl = lx * ly;
+      const bool coarsest = (l == I.nlev - 1);
+      for (int k0 = 0; k0 < Nl; k0 += kBlock) {
+        M.cmap[l].push_back({c, k0});
+        if (!coarsest) M.pmap[l].push_back({c, k0});
+      }
+      if (coarsest) {
+        if (ly == 1) M.cxmap[l].push_back({c, 0});
+        else M.cymap[l].push_back({c, 0});
+      } else {
+        for (int j = 0; j < ly; ++j) M.smap[l][0][j & 1].push_back({c, j});
+        for (int i = 0; i < lx; ++i) M.smap[l][1][i & 1].push_back({c, i});
+      }
+    }
+  }
+  // one device buffer for all maps
+  size_t bytes = 0;
+  auto sz = [&](size_t n, size_t e) { bytes += a256(n * e); };
+  for (int l = 0; l < kMaxLev; ++l) {
+    sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) sz(M.smap[l][d][p].size(), sizeof(LineMap));
+    sz(M.cxmap[l].size(), sizeof(LineMap)); sz(M.cymap[l].size(), sizeof(LineMap));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  if (bytes > d_maps_bytes) {
+    (void)hipFree(d_maps);
+    d_maps = nullptr;
+    HIPCHK(hipMalloc(&d_maps, bytes));
+    d_maps_bytes = bytes;
+  }
+  std::vector<char> host(bytes ? bytes : 1);
+  size_t off = 0;
+  auto put = [&](const void *src, size_t n, size_t e) -> const void * {
+    if (n) std::memcpy(host.data() + off, src, n * e);
+    const void *p = static_cast<char *>(d_maps) + off;
+    off += a256(n * e);
+    return p;
+  };
+  DevMaps dmn{};
+  for (int l = 0; l < kMaxLev; ++l) {
+    dmn.cmap[l] = (const BlkMap *)put(M.cmap[l].data(), M.cmap[l].size(), sizeof(BlkMap));
+    dmn.ncmap[l] = (int)M.cmap[l].size();
+    dmn.pmap[l] = (const BlkMap *)put(M.pmap[l].data(), M.pmap[l].size(), sizeof(BlkMap));
+    dmn.npmap[l] = (int)M.pmap[l].size();
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) {
+        dmn.smap[l][d][p] = (const LineMap *)put(M.smap[l][d][p].data(), M.smap[l][d][p].size(),
+                                                  sizeof(LineMap));
+        dmn.nsmap[l][d][p] = (int)M.smap[l][d][p].size();
+      }
+    dmn.cxmap[l] = (const LineMap *)put(M.cxmap[l].data(), M.cxmap[l].size(), sizeof(LineMap));
+    dmn.ncx[l] = (int)M.cxmap[l].size();
+    dmn.cymap[l] = (const LineMap *)put(M.cymap[l].data(), M.cymap[l].size(), sizeof(LineMap));
+    dmn.ncy[l] = (int)M.cymap[l].size();
+  }
+  dmn.maxlev = maxlev;
+  if (bytes) HIPCHK(hipMemcpy(d_maps, host.data(), bytes, hipMemcpyHostToDevice));
+  if (!table.empty())
+    HIPCHK(hipMemcpy(d_insts, table.data(), table.size() * sizeof(InstDev), hipMemcpyHostToDevice));
+  dm = dmn;
+  part_host.assign(order.size(), -1);   // forces a re-upload of the participation flags
+  dirty = false;
+  return SG_OK;
+}
+
+int Batch::read_counter(int which, int *out) {
+  HIPCHK(hipMemcpyAsync(h_cnt + which, d_cnt + which, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  *out = h_cnt[which];
+  return SG_OK;
+}
+
+void Batch::tbegin(int id) {
+  if (id != time_id) return;
+  if (ev_used + 2 > ev_pool.size()) {
+    for (int k = 0; k < 64; ++k) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) == hipSuccess) ev_pool.push_back(e);
+    }
+  }
+  if (ev_used + 2 <= ev_pool.size()) (void)hipEventRecord(ev_pool[ev_used], stream);
+}
+
+void Batch::tend(int id) {
+  if (id != time_id || ev_used + 2 > ev_pool.size()) return;
+  (void)hipEventRecord(ev_pool[ev_used + 1], stream);
+  ev_used += 2;
+}
+
+int Batch::collect_time() {
+  if (!ev_used) return SG_OK;
+  HIPCHK(hipStreamSynchronize(stream));
+  for (size_t k = 0; k + 1 < ev_used; k += 2) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
+    t_ms += ms;
+    t_launches += 1;
+  }
+  ev_used = 0;
+  return SG_OK;
+}
+
+template <typename T> void Batch::vcycle(int l, int xid, int bid) {
+  const DevMaps &m = dm;
+  const int sm = opts.smoother;
+  const bool pre_x = sm & 1, pre_y = sm & 2, post_x = sm & 4, post_y = sm & 8;
+  const int lid = l == 0 ? 2 : 3;
+  bool zero = true;
+  auto sweep = [&](int dir) {
+    for (int p = 0; p < 2; ++p) {
+      tbegin(lid);
+      launch_line<T>(stream, d_insts, m.smap[l][dir][p], m.nsmap[l][dir][p], l, dir, zero, xid, bid);
+      tend(lid);
+      zero = false;
+    }
+  };
+  if (pre_x) sweep(0);
+  if (pre_y) sweep(1);
+  const bool pre_zero = zero;
+  // exact solve on the coarsest level of the instances that end here
+  tbegin(3);
+  launch_line<T>(stream, d_insts, m.cxmap[l], m.ncx[l], l, 0, true, xid, bid);
+  launch_line<T>(stream, d_insts, m.cymap[l], m.ncy[l], l, 1, true, xid, bid);
+  tend(3);
+  if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
+    tbegin(4);
+    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l, pre_zero, xid, bid);
+    tend(4);
+    vcycle<T>(l + 1, xid, bid);
+    tbegin(5);
+    launch_prolong<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid);
+    tend(5);
+    zero = false;
+    if (post_x) sweep(0);
+    if (post_y) sweep(1);
+  }
+}
+
+template <typename T> void Batch::iteration(bool xstate) {
+  const int nb = dm.ncmap[0];
+  const BlkMap *m0 = dm.cmap[0];
+  tbegin(8); launch_bicg<T>(stream, 0, d_insts, m0, nb, d_cnt, xstate); tend(8);
+  vcycle<T>(0, VPH, VP);
+  tbegin(6); launch_bicg<T>(stream, 1, d_insts, m0, nb, d_cnt, xstate); tend(6);
+  tbegin(8); launch_bicg<T>(stream, 2, d_insts, m0, nb, d_cnt, xstate); tend(8);
+  vcycle<T>(0, VSH, VS);
+  tbegin(7); launch_bicg<T>(stream, 3, d_insts, m0, nb, d_cnt, xstate); tend(7);
+  tbegin(8); launch_bicg<T>(stream, 4, d_insts, m0, nb, d_cnt, xstate); tend(8);
+}
+
+template <typename T> int Batch::krylov(int maxit, bool xstate) {
+  int active = 0, rc;
+  if ((rc = read_counter(0, &active))) return rc;
+  for (int it = 0; it < maxit && active > 0;) {
+    const int chunk = std::min(opts.check_every, maxit - it);
+    for (int c = 0; c < chunk; ++c) iteration<T>(xstate);
+    it += chunk;
+    if ((rc = read_counter(0, &active))) return rc;
+  }
+  return SG_OK;
+}
+
+int Batch::step_once() {
+  int rc;
+  const int n = (int)order.size();
+  if (!n) return SG_OK;
+  std::vector<int> part(n);
+  int any = 0;
+  for (int c = 0; c < n; ++c) {
+    const Slot &s = slots[order[c]];
+    part[c] = (s.target < 0 || s.steps < s.target) ? 1 : 0;
+    any += part[c];
+  }
+  if (!any) return SG_OK;
+  if (part != part_host) {
+    HIPCHK(hipMemcpyAsync(d_part, part.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    part_host = part;
+  }
+  const bool mixed = opts.precision == SG_MIXED;
+  const double tol2 = mixed ? opts.inner_rtol * opts.inner_rtol : opts.rtol * opts.rtol;
+  launch_begin(stream, d_insts, n, d_part, d_cnt, d_cnt + 1, tol2, opts.rtol * opts.rtol,
+               mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
+  const int nb0 = dm.ncmap[0];
+  tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
+  if (mixed) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
+  for (int l = 0; l + 1 < dm.maxlev; ++l) {
+    tbegin(1); launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed); tend(1);
+  }
+  if (!mixed) {
+    launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0, d_cnt);
+    if ((rc = krylov<double>(opts.max_iters, true))) return rc;
+  } else {
+    for (int ref = 0; ref <= opts.max_iters; ++ref) {
+      tbegin(9); launch_ir_residual(stream, d_insts, dm.cmap[0], nb0, d_cnt + 1, ref == 0); tend(9);
+      int oactive = 0;
+      if ((rc = read_counter(1, &oactive))) return rc;
+      if (!oactive) break;
+      launch_ir_inner_init(stream, d_insts, dm.cmap[0], nb0, d_cnt);
+      launch_copy_counter(stream, d_cnt + 1, d_cnt);
+      if ((rc = krylov<float>(opts.max_inner, false))) return rc;
+      launch_ir_update(stream, d_insts, dm.cmap[0], nb0);
+    }
+  }
+  tbegin(10); launch_finalize(stream, d_insts, dm.cmap[0], nb0); tend(10);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h_scal.data(), d_scal, sizeof(KScal) * slots.size(), hipMemcpyDeviceToHost,
+                        stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if ((rc = collect_time())) return rc;
+  int worst = SG_OK;
+  for (int c = 0; c < n; ++c) {
+    if (!part[c]) continue;
+    Slot &s = slots[order[c]];
+    const KScal &k = h_scal[order[c]];
+    s.steps += 1;
+    s.st.steps = s.steps;
+    s.st.last_iters = k.titers;
+    s.st.total_iters += k.titers;
+    s.st.last_refinements = mixed ? k.orefs : 0;
+    s.st.last_rel_residual = k.true_rel;
+    s.st.min_f = k.minf;
+    s.st.clamp_count += k.clamps;
+    const bool failed = mixed ? (k.ostatus == ST_FAIL) : (k.status == ST_FAIL);
+    s.st.status = failed ? SG_ERR_NOCONV : SG_OK;
+    if (failed) worst = SG_ERR_NOCONV;
+  }
+  return worst;
+}
+
+int Batch::step(int nsteps, const int64_t *targets, int ntargets) {
+  int rc;
+  if (nsteps < 0) return fail(SG_ERR_ARG, "nsteps must be >= 0");
+  if (dirty && (rc = rebuild())) return rc;
+  for (size_t k = 0; k < slots.size(); ++k)
+    slots[k].target = (targets && (int)k < ntargets) ? targets[k] : -1;
+  int worst = SG_OK;
+  for (int s = 0; s < nsteps; ++s) {
+    rc = step_once();
+    if (rc == SG_ERR_NOCONV) { worst = rc; g_err = "Krylov solve did not converge"; continue; }
+    if (rc) return rc;
+  }
+  return worst;
+}
+
+int Batch::scratch(size_t n) {
+  if (n <= scratch_elems) return SG_OK;
+  (void)hipFree(d_scratch);
+  d_scratch = nullptr;
+  HIPCHK(hipMalloc(&d_scratch, n * sizeof(double)));
+  scratch_elems = n;
+  return SG_OK;
+}
+
+int Batch::get_f(int slot, double *out) {
+  if (slot < 0 || slot >= (int)slots.size() || !slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  if (!out) return fail(SG_ERR_ARG, "out must not be null");
+  const Slot &s = slots[slot];
+  int rc;
+  if ((rc = scratch((size_t)s.nx * s.ny))) return rc;
+  launch_to_imajor(stream, s.dev.f, d_scratch, s.nx, s.ny);
+  HIPCHK(hipMemcpyAsync(out, d_scratch, (size_t)s.nx * s.ny * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SG_OK;
+}
+
+int Batch::set_f(int slot, const double *f, int64_t istep) {
+  if (slot < 0 || slot >= (int)slots.size() || !slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  if (!f) return fail(SG_ERR_ARG, "f must not be null");
+  Slot &s = slots[slot];
+  const size_t N = (size_t)s.nx * s.ny;
+  for (size_t q = 0; q < N; ++q)
+    if (!std::isfinite(f[q])) return fail(SG_ERR_ARG, "f: non-finite value");
+  int rc;
+  if ((rc = scratch(N))) return rc;
+  HIPCHK(hipMemcpyAsync(d_scratch, f, N * 8, hipMemcpyHostToDevice, stream));
+  launch_to_korder(stream, d_scratch, s.dev.f, s.nx, s.ny);
+  HIPCHK(hipStreamSynchronize(stream));
+  s.steps = istep;
+  s.st.steps = istep;
+  return SG_OK;
+}
+
+int Batch::debug_assemble(int slot, double *bands, double *R) {
+  if (slot < 0 || slot >= (int)slots.size() || !slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  int rc;
+  if (dirty && (rc = rebuild())) return rc;
+  const int n = (int)order.size();
+  std::vector<int> part(n, 0);
+  for (int c = 0; c < n; ++c) part[c] = order[c] == slot;
+  HIPCHK(hipMemcpyAsync(d_part, part.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+  part_host = part;
+  launch_begin(stream, d_insts, n, d_part, d_cnt, d_cnt + 1, 0, 0, 1, 1);
+  launch_assemble(stream, d_insts, dm.cmap[0], dm.ncmap[0]);
+  HIPCHK(hipGetLastError());
+  const Slot &s = slots[slot];
+  const size_t N = (size_t)s.nx * s.ny;
+  for (int b = 0; b < 5; ++b)
+    HIPCHK(hipMemcpyAsync(bands + b * N, s.dev.band64[b], N * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(R, s.dev.R, N * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SG_OK;
+}
+
+}  // namespace sg
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using sg::Batch;
+using sg::fail;
+
+struct sg_batch { Batch b; };
+struct sg_solver { sg_batch *bt; int slot; };
+
+extern "C" {
+
+int sg_abi_version(void) { return SG_ABI_VERSION; }
+const char *sg_last_error(void) { return sg::g_err.c_str(); }
+int sg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return n;
+}
+
+size_t sg_instance_bytes(int nx, int ny, int precision) {
+  if (nx < 1 || ny < 1) return 0;
+  sg::InstDev tmp{};
+  return sg::carve(0, nx, ny, precision == SG_MIXED, tmp);
+}
+
+int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
+  if (!out) return fail(SG_ERR_ARG, "out must not be null");
+  *out = nullptr;
+  std::unique_ptr<sg_batch> b(new sg_batch);
+  int rc = b->b.init(opts, budget);
+  if (rc) return rc;
+  *out = b.release();
+  return SG_OK;
+}
+
+int sg_batch_add(sg_batch *b, const sg_grid *g, const sg_coeffs *c, const sg_bc *bc,
+                 const double *f0, double dt, int *slot) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  return b->b.add(g, c, bc, f0, dt, slot);
+}
+int sg_batch_remove(sg_batch *b, int slot) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  return b->b.remove(slot);
+}
+int sg_batch_step(sg_batch *b, int nsteps) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  return b->b.step(nsteps, nullptr, 0);
+}
+int sg_batch_step_until(sg_batch *b, const int64_t *targets, int nslots, int nsteps) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  if (!targets) return fail(SG_ERR_ARG, "targets must not be null");
+  return b->b.step(nsteps, targets, nslots);
+}
+int sg_batch_get_f(sg_batch *b, int slot, double *out) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  return b->b.get_f(slot, out);
+}
+int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  return b->b.set_f(slot, f, istep);
+}
+int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  if (slot < 0 || slot >= (int)b->b.slots.size() || !b->b.slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot");
+  if (side < 0 || side > 3) return fail(SG_ERR_ARG, "side must be 0..3");
+  auto &s = b->b.slots[slot];
+  if (!(s.dev.bcmask & (1 << side))) return fail(SG_ERR_ARG, "side is not Dirichlet");
+  return b->b.upload_bc(s, side, values);
+}
+int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out) {
+  if (!b || !out) return fail(SG_ERR_ARG, "null argument");
+  if (slot < 0 || slot >= (int)b->b.slots.size() || !b->b.slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot");
+  *out = b->b.slots[slot].st;
+  return SG_OK;
+}
+int sg_batch_size(const sg_batch *b) {
+  if (!b) return 0;
+  int n = 0;
+  for (auto &s : b->b.slots) n += s.used;
+  return n;
+}
+size_t sg_batch_bytes_in_use(const sg_batch *b) { return b ? b->b.in_use : 0; }
+void sg_batch_destroy(sg_batch *b) { delete b; }
+
+int sg_batch_time_kernel(sg_batch *b, int kernel_id) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  if (kernel_id >= sg::kNumKernelIds) return fail(SG_ERR_ARG, "unknown kernel id");
+  b->b.time_id = kernel_id;
+  b->b.t_ms = 0;
+  b->b.t_launches = 0;
+  return SG_OK;
+}
+int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  if (total_ms) *total_ms = b->b.t_ms;
+  if (launches) *launches = b->b.t_launches;
+  if (bytes) *bytes = 0;
+  return SG_OK;
+}
+const char *sg_kernel_name(int id) {
+  return (id >= 0 && id < sg::kNumKernelIds) ? sg::kKernelNames[id] : "";
+}
+
+int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, const double *f0,
+              double dt, const sg_opts *opts, sg_solver **out) {
+  if (!out) return fail(SG_ERR_ARG, "out must not be null");
+  *out = nullptr;
+  sg_batch *b = nullptr;
+  int rc = sg_batch_create(opts, 0, &b);
+  if (rc) return rc;
+  int slot = -1;
+  rc = b->b.add(grid, coeffs, bc, f0, dt, &slot);
+  if (rc) { delete b; return rc; }
+  *out = new sg_solver{b, slot};
+  return SG_OK;
+}
+int sg_set_bc_values(sg_solver *h, int side, const double *values) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  return sg_batch_set_bc_values(h->bt, h->slot, side, values);
+}
+int sg_set_coeffs(sg_solver *h, const sg_coeffs *c) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  int rc = sg::check_coeffs(c);
+  if (rc) return rc;
+  return h->bt->b.upload_coeffs(h->bt->b.slots[h->slot], c);
+}
+int sg_step(sg_solver *h, int nsteps) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  return h->bt->b.step(nsteps, nullptr, 0);
+}
+int sg_get_f(sg_solver *h, double *out) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  return h->bt->b.get_f(h->slot, out);
+}
+int sg_set_f(sg_solver *h, const double *f, int64_t istep) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  return h->bt->b.set_f(h->slot, f, istep);
+}
+double sg_time(const sg_solver *h) {
+  if (!h) return 0.0;
+  const auto &s = h->bt->b.slots[h->slot];
+  return (double)s.steps * s.dt;   // Solver::t(), source/Solver.h:23
+}
+int sg_stats(const sg_solver *h, sg_step_stats *out) {
+  if (!h) return fail(SG_ERR_ARG, "null handle");
+  return sg_batch_stats(h->bt, h->slot, out);
+}
+void sg_destroy(sg_solver *h) {
+  if (!h) return;
+  delete h->bt;
+  delete h;
+}
+int sg_debug_assemble(sg_solver *h, double *bands5, double *R) {
+  if (!h || !bands5 || !R) return fail(SG_ERR_ARG, "null argument");
+  return h->bt->b.debug_assemble(h->slot, bands5, R);
+}
+int sg_solver_batch(sg_solver *h, sg_batch **out) {
+  if (!h || !out) return fail(SG_ERR_ARG, "null argument");
+  *out = h->bt;
+  return SG_OK;
+}
+
+}  // extern "C"

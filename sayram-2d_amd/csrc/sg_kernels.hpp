@@ -1,0 +1,35 @@
+// sg_kernels.hpp -- launch wrappers of the gfx950 kernels (sg_kernels.hip).
+#pragma once
+#include "sg_internal.hpp"
+
+namespace sg {
+
+void launch_begin(hipStream_t s, const InstDev *d, int n, const int *part, int *active,
+                  int *oactive, double tol2, double otol2, int maxit, int omaxit);
+void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
+template <typename T>
+void launch_line(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
+                 bool zero, int xid, int bid);
+template <typename T>
+void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
+                     int xid, int bid);
+template <typename T>
+void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
+                    int xid);
+void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active);
+// which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update
+template <typename T>
+void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, int *active,
+                 bool xstate);
+void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *oactive,
+                        int first);
+void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active);
+void launch_copy_counter(hipStream_t s, const int *src, int *dst);
+void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_to_korder(hipStream_t s, const double *src, double *dst, int nx, int ny);
+void launch_to_imajor(hipStream_t s, const double *src, double *dst, int nx, int ny);
+
+}  // namespace sg
