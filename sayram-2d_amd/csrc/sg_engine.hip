@@ -82,12 +82,20 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
     L.nx = dims[l].nx; L.ny = dims[l].ny;
     const size_t Nl = (size_t)L.nx * L.ny;
     if (l == 0) {
-      for (int b = 0; b < 5; ++b) L.band[b] = mixed ? take(N * T) : (void *)I.band64[b];
+      for (int b = 0; b < 5; ++b) {
+        L.band64[b] = I.band64[b];
+        L.band[b] = mixed ? take(N * T) : (void *)I.band64[b];
+      }
     } else {
-      for (int b = 0; b < 9; ++b) L.band[b] = take(Nl * T);
+      for (int b = 0; b < 9; ++b) {
+        L.band64[b] = (double *)take(Nl * D);
+        L.band[b] = mixed ? take(Nl * T) : (void *)L.band64[b];
+      }
       L.x = take(Nl * T);
       L.b = take(Nl * T);
     }
+    if (l + 1 < I.nlev)
+      for (int q = 0; q < 4; ++q) L.pw[q] = take(Nl * T);
     L.cp = take(Nl * T);
     L.dp = take(Nl * T);
   }
@@ -549,7 +557,8 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   tend(3);
   if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
     tbegin(4);
-    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l, pre_zero, xid, bid);
+    launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid, bid);
+    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l);
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
@@ -609,7 +618,10 @@ int Batch::step_once() {
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
   if (mixed) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
   for (int l = 0; l + 1 < dm.maxlev; ++l) {
-    tbegin(1); launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed); tend(1);
+    tbegin(1);
+    launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, mixed);
+    launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed);
+    tend(1);
   }
   if (!mixed) {
     launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0, d_cnt);

@@ -30,6 +30,7 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   double bnorm2, rnorm2, tol2;
   double scale;          // mixed: norm of the FP64 residual being corrected
   double onorm2, obnorm2, otol2;   // mixed: outer (FP64) residual norms
+  double oprev2;         // mixed: residual norm^2 of the previous refinement
   double true_rel;       // finalize: ||R - M f|| / ||R||
   double minf;           // finalize: min f before clamping
   int iters, status;     // Krylov iteration count and status of this solve
@@ -41,7 +42,9 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
 
 struct Lev {            // one multigrid level of one instance
   int nx, ny;
-  void *band[9];        // precision of the V-cycle (double or float)
+  double *band64[9];    // FP64 operator (level 0: the 5 assembled bands)
+  void *band[9];        // V-cycle precision copy (aliases band64 in FP64 mode)
+  void *pw[4];          // prolongation weights to the next coarser level
   void *x, *b, *cp, *dp;
 };
 

@@ -313,19 +313,94 @@ __global__ void __launch_bounds__(kBlock) k_bands_to_f32(const InstDev *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Galerkin coarse operators: A_{l+1} = R A_l S P, R = 2x2 aggregation (sum of
-// the four children), P = cell-centred bilinear (constant next to the
-// boundary), S = diag(A_0)^-1 on level 0 (the coarse space is the column-
-// scaled one) and the identity below. The result is 9-point on every level.
+// Black-box multigrid hierarchy (operator-dependent interpolation on a
+// vertex-centred coarsening: coarse points are the even (i, j) of the level).
+// P is rebuilt from the operator every step, R = P^T, and the coarse operator
+// A_{l+1} = P^T A_l P is 9-point; the whole chain is FP64.
+//
+// Prolongation weights of fine point (i, j) refer to the coarse points
+// (i>>1, j>>1), (i>>1 + 1, j>>1), (i>>1, j>>1 + 1), (i>>1 + 1, j>>1 + 1).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void pweights(int g, int nc, int &p0, double &w0, int &p1, double &w1) {
-  p0 = g >> 1;
-  p1 = (g & 1) ? p0 + 1 : p0 - 1;
-  if (p1 >= 0 && p1 < nc) { w0 = 0.75; w1 = 0.25; }
-  else { w0 = 1.0; w1 = 0.0; p1 = p0; }
+struct St9 { double c, w, e, s, n, sw, se, nw, ne; };
+
+template <int NB>
+__device__ __forceinline__ St9 load_st(const Lev &L, size_t K) {
+  St9 a;
+  a.c = L.band64[BC][K]; a.w = L.band64[BW][K]; a.e = L.band64[BE][K];
+  a.s = L.band64[BS][K]; a.n = L.band64[BN][K];
+  if (NB == 9) {
+    a.sw = L.band64[BSW][K]; a.se = L.band64[BSE][K];
+    a.nw = L.band64[BNW][K]; a.ne = L.band64[BNE][K];
+  } else {
+    a.sw = a.se = a.nw = a.ne = 0.0;
+  }
+  return a;
 }
 
-template <typename Tin, typename Tout, int NB, bool SCALE>
+__device__ __forceinline__ double safe_den(double d, double c) {
+  return (d > 0.0 && d < INFINITY) ? d : c;
+}
+
+// x-edge point (i odd, j even): interpolate from west/east coarse points with
+// the stencil collapsed along y.
+template <int NB>
+__device__ __forceinline__ void w_xedge(const Lev &L, size_t K, int i, double &wW, double &wE) {
+  const St9 a = load_st<NB>(L, K);
+  const double den = safe_den(a.c + a.n + a.s, a.c);
+  wW = -(a.w + a.nw + a.sw) / den;
+  wE = (i + 1 < L.nx) ? -(a.e + a.ne + a.se) / den : 0.0;
+}
+
+// y-edge point (i even, j odd): from south/north, stencil collapsed along x.
+template <int NB>
+__device__ __forceinline__ void w_yedge(const Lev &L, size_t K, int j, double &wS, double &wN) {
+  const St9 a = load_st<NB>(L, K);
+  const double den = safe_den(a.c + a.w + a.e, a.c);
+  wS = -(a.s + a.sw + a.se) / den;
+  wN = (j + 1 < L.ny) ? -(a.n + a.nw + a.ne) / den : 0.0;
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__ insts,
+                                                    const BlkMap *__restrict__ map, int l) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (I.sc->ostatus == ST_IDLE) return;
+  const Lev &L = I.lev[l];
+  const int nx = L.nx, ny = L.ny;
+  const int K = m.k0 + threadIdx.x;
+  if (K >= nx * ny) return;
+  const int j = K / nx, i = K - j * nx;
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+  const bool io = i & 1, jo = j & 1;
+  if (!io && !jo) {
+    w0 = 1.0;
+  } else if (io && !jo) {
+    w_xedge<NB>(L, K, i, w0, w1);
+  } else if (!io && jo) {
+    w_yedge<NB>(L, K, j, w0, w2);
+  } else {
+    const St9 a = load_st<NB>(L, K);
+    double sW = 0, nW = 0, sE = 0, nE = 0, wS = 0, eS = 0, wN = 0, eN = 0;
+    w_yedge<NB>(L, K - 1, j, sW, nW);
+    if (i + 1 < nx) w_yedge<NB>(L, K + 1, j, sE, nE);
+    w_xedge<NB>(L, K - nx, i, wS, eS);
+    if (j + 1 < ny) w_xedge<NB>(L, K + nx, i, wN, eN);
+    const double c = a.c;
+    w0 = -(a.w * sW + a.s * wS + a.sw) / c;
+    w1 = -(a.e * sE + a.s * eS + a.se) / c;
+    w2 = -(a.w * nW + a.n * wN + a.nw) / c;
+    w3 = -(a.e * nE + a.n * eN + a.ne) / c;
+  }
+  ptr<T>(L.pw[0])[K] = (T)w0;
+  ptr<T>(L.pw[1])[K] = (T)w1;
+  ptr<T>(L.pw[2])[K] = (T)w2;
+  ptr<T>(L.pw[3])[K] = (T)w3;
+}
+
+// A_{l+1}(C, C') = sum_f sum_g P(f, C) A_l(f, g) P(g, C'); one work-item per
+// coarse point, whose support is the 3x3 fine block around (2I, 2J).
+template <typename T, int NB, bool COPY>
 __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ insts,
                                                const BlkMap *__restrict__ map, int l) {
   const BlkMap m = map[blockIdx.x];
@@ -337,9 +412,10 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   const int KC = m.k0 + threadIdx.x;
   if (KC >= cx * cy) return;
   const int J = KC / cx, Ic = KC - J * cx;
-  const Tin *B[9];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) B[b] = SCALE ? (const Tin *)I.band64[b] : cptr<Tin>(F.band[b]);
+  const T *__restrict__ pw0 = cptr<T>(F.pw[0]);
+  const T *__restrict__ pw1 = cptr<T>(F.pw[1]);
+  const T *__restrict__ pw2 = cptr<T>(F.pw[2]);
+  const T *__restrict__ pw3 = cptr<T>(F.pw[3]);
   double acc[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -347,44 +423,43 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
     for (int c = 0; c < 3; ++c) acc[a][c] = 0.0;
   const int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
   const int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
-#pragma unroll
-  for (int cj = 0; cj < 2; ++cj) {
-    const int fj = 2 * J + cj;
-    if (fj >= fy) continue;
-#pragma unroll
-    for (int ci = 0; ci < 2; ++ci) {
-      const int fi = 2 * Ic + ci;
-      if (fi >= fx) continue;
+  for (int fj = 2 * J - 1; fj <= 2 * J + 1; ++fj) {
+    if (fj < 0 || fj >= fy) continue;
+    const int dJ = J - (fj >> 1);
+    for (int fi = 2 * Ic - 1; fi <= 2 * Ic + 1; ++fi) {
+      if (fi < 0 || fi >= fx) continue;
+      const int dI = Ic - (fi >> 1);
       const size_t kf = (size_t)fj * fx + fi;
+      const int slot = dI + 2 * dJ;
+      const double wf = (double)(slot == 0 ? pw0[kf] : slot == 1 ? pw1[kf] : slot == 2 ? pw2[kf] : pw3[kf]);
+      if (wf == 0.0) continue;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int gi = fi + di_[b], gj = fj + dj_[b];
         if (gi < 0 || gi >= fx || gj < 0 || gj >= fy) continue;
-        double a = (double)B[b][kf];
-        if (SCALE) a = a / (double)I.band64[BC][(size_t)gj * fx + gi];
-        int px0, px1, py0, py1;
-        double wx0, wx1, wy0, wy1;
-        pweights(gi, cx, px0, wx0, px1, wx1);
-        pweights(gj, cy, py0, wy0, py1, wy1);
-        acc[py0 - J + 1][px0 - Ic + 1] += a * wx0 * wy0;
-        acc[py0 - J + 1][px1 - Ic + 1] += a * wx1 * wy0;
-        acc[py1 - J + 1][px0 - Ic + 1] += a * wx0 * wy1;
-        acc[py1 - J + 1][px1 - Ic + 1] += a * wx1 * wy1;
+        const double av = F.band64[b][kf];
+        if (av == 0.0) continue;
+        const double wa = wf * av;
+        const size_t kg = (size_t)gj * fx + gi;
+        const int oi = (gi >> 1) - Ic + 1, oj = (gj >> 1) - J + 1;
+        const double wg[4] = {(double)pw0[kg], (double)pw1[kg], (double)pw2[kg], (double)pw3[kg]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (wg[q] == 0.0) continue;
+          const int ai = oi + (q & 1), aj = oj + (q >> 1);
+          if (ai < 0 || ai > 2 || aj < 0 || aj > 2) continue;
+          acc[aj][ai] += wa * wg[q];
+        }
       }
     }
   }
-  Tout *O[9];
+  const double out[9] = {acc[1][1], acc[1][0], acc[1][2], acc[0][1], acc[2][1],
+                         acc[0][0], acc[0][2], acc[2][0], acc[2][2]};
 #pragma unroll
-  for (int b = 0; b < 9; ++b) O[b] = ptr<Tout>(Cl.band[b]);
-  O[BC][KC] = (Tout)acc[1][1];
-  O[BW][KC] = (Tout)acc[1][0];
-  O[BE][KC] = (Tout)acc[1][2];
-  O[BS][KC] = (Tout)acc[0][1];
-  O[BN][KC] = (Tout)acc[2][1];
-  O[BSW][KC] = (Tout)acc[0][0];
-  O[BSE][KC] = (Tout)acc[0][2];
-  O[BNW][KC] = (Tout)acc[2][0];
-  O[BNE][KC] = (Tout)acc[2][2];
+  for (int b = 0; b < 9; ++b) {
+    Cl.band64[b][KC] = out[b];
+    if (COPY) ptr<float>(Cl.band[b])[KC] = (float)out[b];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -512,12 +587,28 @@ __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j,
   return s;
 }
 
-// b_{l+1} = R (b_l - A_l x_l): one work-item per coarse cell sums the
-// residuals of its (up to) four children. ZERO: x_l = 0.
+// r_l = b_l - A_l x_l into the level's scratch (the line-solve buffer cp is
+// free between the pre- and post-smoothing sweeps). ZERO: x_l = 0.
 template <typename T, int NB, bool ZERO>
-__global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
+__global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (I.sc->status != ST_ACTIVE) return;
+  const Lev &F = I.lev[l];
+  const int K = m.k0 + threadIdx.x;
+  if (K >= F.nx * F.ny) return;
+  const int j = K / F.nx, i = K - j * F.nx;
+  T *x; const T *b;
+  level_xb<T>(I, l, xid, bid, x, b);
+  ptr<T>(F.cp)[K] = ZERO ? b[K] : b[K] - stencil_apply<T, NB>(F, K, i, j, x);
+}
+
+// b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
+                                                    const BlkMap *__restrict__ map, int l) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (I.sc->status != ST_ACTIVE) return;
@@ -526,25 +617,22 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   const int KC = m.k0 + threadIdx.x;
   if (KC >= Cl.nx * Cl.ny) return;
   const int J = KC / Cl.nx, Ic = KC - J * Cl.nx;
-  T *x; const T *b;
-  level_xb<T>(I, l, xid, bid, x, b);
+  const T *__restrict__ r = cptr<T>(F.cp);
   T s = T(0);
-#pragma unroll
-  for (int cj = 0; cj < 2; ++cj) {
-    const int fj = 2 * J + cj;
-    if (fj >= F.ny) continue;
-#pragma unroll
-    for (int ci = 0; ci < 2; ++ci) {
-      const int fi = 2 * Ic + ci;
-      if (fi >= F.nx) continue;
-      const size_t K = (size_t)fj * F.nx + fi;
-      s += ZERO ? b[K] : b[K] - stencil_apply<T, NB>(F, K, fi, fj, x);
+  for (int fj = 2 * J - 1; fj <= 2 * J + 1; ++fj) {
+    if (fj < 0 || fj >= F.ny) continue;
+    const int dJ = J - (fj >> 1);
+    for (int fi = 2 * Ic - 1; fi <= 2 * Ic + 1; ++fi) {
+      if (fi < 0 || fi >= F.nx) continue;
+      const int slot = (Ic - (fi >> 1)) + 2 * dJ;
+      const size_t kf = (size_t)fj * F.nx + fi;
+      s += cptr<T>(F.pw[slot])[kf] * r[kf];
     }
   }
   ptr<T>(Cl.b)[KC] = s;
 }
 
-// x_l += S P x_{l+1} (S = diag(A_0)^-1 on level 0). ZERO: x_l = S P x_{l+1}.
+// x_l += P x_{l+1}. ZERO: x_l = P x_{l+1}.
 template <typename T, bool ZERO>
 __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ insts,
                                                    const BlkMap *__restrict__ map, int l,
@@ -557,19 +645,18 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
   const int K = m.k0 + threadIdx.x;
   if (K >= F.nx * F.ny) return;
   const int j = K / F.nx, i = K - j * F.nx;
+  const int I0 = i >> 1, J0 = j >> 1;
+  const int I1 = I0 + 1 < Cl.nx ? I0 + 1 : I0, J1 = J0 + 1 < Cl.ny ? J0 + 1 : J0;
   const T *__restrict__ xc = cptr<T>(Cl.x);
-  int px0, px1, py0, py1;
-  double wx0, wx1, wy0, wy1;
-  pweights(i, Cl.nx, px0, wx0, px1, wx1);
-  pweights(j, Cl.ny, py0, wy0, py1, wy1);
-  const size_t r0 = (size_t)py0 * Cl.nx, r1 = (size_t)py1 * Cl.nx;
-  T e = (T)(wx0 * wy0) * xc[r0 + px0] + (T)(wx1 * wy0) * xc[r0 + px1] +
-        (T)(wx0 * wy1) * xc[r1 + px0] + (T)(wx1 * wy1) * xc[r1 + px1];
-  if (l == 0) e = e / cptr<T>(F.band[BC])[K];
+  const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
+  // weights of parents outside the coarse grid are zero
+  const T e = cptr<T>(F.pw[0])[K] * xc[r0 + I0] + cptr<T>(F.pw[1])[K] * xc[r0 + I1] +
+              cptr<T>(F.pw[2])[K] * xc[r1 + I0] + cptr<T>(F.pw[3])[K] * xc[r1 + I1];
   T *x = l == 0 ? ptr<T>(I.kv[xid]) : ptr<T>(F.x);
   x[K] = ZERO ? e : x[K] + e;
 }
 
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // BiCGSTAB (right-preconditioned), precision T for vectors and sums; the
 // operator is the level-0 5-band matrix of the V-cycle precision.
@@ -776,7 +863,12 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restric
     if (first) s.obnorm2 = tot[1];
     s.onorm2 = tot[0];
     s.scale = sqrt(tot[0]);
-    if (tot[0] <= s.otol2 * s.obnorm2) {
+    // Converged; or stalled at the FP64 rounding floor of R - M x (the
+    // residual stopped halving within 100x of the tolerance).
+    const bool conv = tot[0] <= s.otol2 * s.obnorm2;
+    const bool floor = !first && tot[0] > 0.25 * s.oprev2 && tot[0] <= 1e4 * s.otol2 * s.obnorm2;
+    s.oprev2 = tot[0];
+    if (conv || floor) {
       s.ostatus = ST_DONE;
       s.status = ST_DONE;
       atomicSub(oactive, 1);
@@ -926,14 +1018,24 @@ void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
 void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_bands_to_f32<<<nb, kBlock, 0, s>>>(d, m);
 }
+void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed) {
+  if (!nb) return;
+  if (l == 0) {
+    if (mixed) k_pweights<float, 5><<<nb, kBlock, 0, s>>>(d, m, l);
+    else k_pweights<double, 5><<<nb, kBlock, 0, s>>>(d, m, l);
+  } else {
+    if (mixed) k_pweights<float, 9><<<nb, kBlock, 0, s>>>(d, m, l);
+    else k_pweights<double, 9><<<nb, kBlock, 0, s>>>(d, m, l);
+  }
+}
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed) {
   if (!nb) return;
   if (l == 0) {
-    if (mixed) k_rap<double, float, 5, true><<<nb, kBlock, 0, s>>>(d, m, l);
-    else k_rap<double, double, 5, true><<<nb, kBlock, 0, s>>>(d, m, l);
+    if (mixed) k_rap<float, 5, true><<<nb, kBlock, 0, s>>>(d, m, l);
+    else k_rap<double, 5, false><<<nb, kBlock, 0, s>>>(d, m, l);
   } else {
-    if (mixed) k_rap<float, float, 9, false><<<nb, kBlock, 0, s>>>(d, m, l);
-    else k_rap<double, double, 9, false><<<nb, kBlock, 0, s>>>(d, m, l);
+    if (mixed) k_rap<float, 9, true><<<nb, kBlock, 0, s>>>(d, m, l);
+    else k_rap<double, 9, false><<<nb, kBlock, 0, s>>>(d, m, l);
   }
 }
 
@@ -959,21 +1061,28 @@ template void launch_line<float>(hipStream_t, const InstDev *, const LineMap *, 
                                  bool, int, int);
 
 template <typename T>
-void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
+void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                      int xid, int bid) {
   if (!nb) return;
   if (l == 0) {
-    if (zero) k_restrict<T, 5, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_restrict<T, 5, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    if (zero) k_residual<T, 5, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    else k_residual<T, 5, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
   } else {
-    if (zero) k_restrict<T, 9, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_restrict<T, 9, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    if (zero) k_residual<T, 9, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    else k_residual<T, 9, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
   }
 }
-template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int,
-                                      bool, int, int);
-template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int,
-                                     bool, int, int);
+template void launch_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
+                                      int, int);
+template void launch_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
+                                     int, int);
+
+template <typename T>
+void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l) {
+  if (nb) k_restrict<T><<<nb, kBlock, 0, s>>>(d, m, l);
+}
+template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int);
+template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int);
 
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,

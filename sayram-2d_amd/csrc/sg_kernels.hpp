@@ -8,13 +8,16 @@ void launch_begin(hipStream_t s, const InstDev *d, int n, const int *part, int *
                   int *oactive, double tol2, double otol2, int maxit, int omaxit);
 void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
                  bool zero, int xid, int bid);
 template <typename T>
-void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
+void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                      int xid, int bid);
+template <typename T>
+void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l);
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid);
