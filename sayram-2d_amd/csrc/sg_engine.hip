@@ -96,8 +96,12 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
     }
     if (l + 1 < I.nlev)
       for (int q = 0; q < 4; ++q) L.pw[q] = take(Nl * T);
+    for (int d = 0; d < 2; ++d) {
+      L.linv[d] = take(Nl * T);
+      L.lup[d] = take(Nl * T);
+    }
     L.cp = take(Nl * T);
-    L.dp = take(Nl * T);
+    L.dp = nullptr;
   }
   return off;
 }
@@ -118,6 +122,7 @@ struct Slot {
 struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
   std::vector<LineMap> smap[kMaxLev][2][2], cxmap[kMaxLev], cymap[kMaxLev];
+  std::vector<LineGrp> gmap[kMaxLev][2][2], gcx[kMaxLev], gcy[kMaxLev];
 };
 
 struct DevMaps {   // offsets into one device buffer
@@ -125,6 +130,8 @@ struct DevMaps {   // offsets into one device buffer
   int ncmap[kMaxLev], npmap[kMaxLev];
   const LineMap *smap[kMaxLev][2][2], *cxmap[kMaxLev], *cymap[kMaxLev];
   int nsmap[kMaxLev][2][2], ncx[kMaxLev], ncy[kMaxLev];
+  const LineGrp *gmap[kMaxLev][2][2], *gcx[kMaxLev], *gcy[kMaxLev];
+  int ngmap[kMaxLev][2][2], ngcx[kMaxLev], ngcy[kMaxLev];
   int maxlev;
 };
 
@@ -203,7 +210,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 2;
-  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 15;
+  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
 }
 
 int Batch::init(const sg_opts *o, size_t b) {
@@ -437,11 +444,20 @@ int Batch::rebuild() {
         if (!coarsest) M.pmap[l].push_back({c, k0});
       }
       if (coarsest) {
-        if (ly == 1) M.cxmap[l].push_back({c, 0});
-        else M.cymap[l].push_back({c, 0});
+        if (ly == 1) { M.cxmap[l].push_back({c, 0}); M.gcx[l].push_back({c, 0, 1, 1}); }
+        else { M.cymap[l].push_back({c, 0}); M.gcy[l].push_back({c, 0, 1, 1}); }
       } else {
         for (int j = 0; j < ly; ++j) M.smap[l][0][j & 1].push_back({c, j});
         for (int i = 0; i < lx; ++i) M.smap[l][1][i & 1].push_back({c, i});
+        // groups of up to 64 lines of one colour
+        for (int d = 0; d < 2; ++d) {
+          const int nl = d == 0 ? ly : lx;
+          for (int p = 0; p < 2; ++p)
+            for (int f = p; f < nl; f += 2 * kLineBlock) {
+              const int cntl = std::min(kLineBlock, (nl - f + 1) / 2);
+              M.gmap[l][d][p].push_back({c, f, cntl, 2});
+            }
+        }
       }
     }
   }
@@ -453,6 +469,9 @@ int Batch::rebuild() {
     for (int d = 0; d < 2; ++d)
       for (int p = 0; p < 2; ++p) sz(M.smap[l][d][p].size(), sizeof(LineMap));
     sz(M.cxmap[l].size(), sizeof(LineMap)); sz(M.cymap[l].size(), sizeof(LineMap));
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) sz(M.gmap[l][d][p].size(), sizeof(LineGrp));
+    sz(M.gcx[l].size(), sizeof(LineGrp)); sz(M.gcy[l].size(), sizeof(LineGrp));
   }
   HIPCHK(hipStreamSynchronize(stream));
   if (bytes > d_maps_bytes) {
@@ -485,6 +504,16 @@ int Batch::rebuild() {
     dmn.ncx[l] = (int)M.cxmap[l].size();
     dmn.cymap[l] = (const LineMap *)put(M.cymap[l].data(), M.cymap[l].size(), sizeof(LineMap));
     dmn.ncy[l] = (int)M.cymap[l].size();
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) {
+        dmn.gmap[l][d][p] = (const LineGrp *)put(M.gmap[l][d][p].data(), M.gmap[l][d][p].size(),
+                                                  sizeof(LineGrp));
+        dmn.ngmap[l][d][p] = (int)M.gmap[l][d][p].size();
+      }
+    dmn.gcx[l] = (const LineGrp *)put(M.gcx[l].data(), M.gcx[l].size(), sizeof(LineGrp));
+    dmn.ngcx[l] = (int)M.gcx[l].size();
+    dmn.gcy[l] = (const LineGrp *)put(M.gcy[l].data(), M.gcy[l].size(), sizeof(LineGrp));
+    dmn.ngcy[l] = (int)M.gcy[l].size();
   }
   dmn.maxlev = maxlev;
   if (bytes) HIPCHK(hipMemcpy(d_maps, host.data(), bytes, hipMemcpyHostToDevice));
@@ -542,7 +571,7 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
       tbegin(lid);
-      launch_line<T>(stream, d_insts, m.smap[l][dir][p], m.nsmap[l][dir][p], l, dir, zero, xid, bid);
+      launch_line<T>(stream, d_insts, m.gmap[l][dir][p], m.ngmap[l][dir][p], l, dir, zero, xid, bid);
       tend(lid);
       zero = false;
     }
@@ -552,8 +581,8 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const bool pre_zero = zero;
   // exact solve on the coarsest level of the instances that end here
   tbegin(3);
-  launch_line<T>(stream, d_insts, m.cxmap[l], m.ncx[l], l, 0, true, xid, bid);
-  launch_line<T>(stream, d_insts, m.cymap[l], m.ncy[l], l, 1, true, xid, bid);
+  launch_line<T>(stream, d_insts, m.gcx[l], m.ngcx[l], l, 0, true, xid, bid);
+  launch_line<T>(stream, d_insts, m.gcy[l], m.ngcy[l], l, 1, true, xid, bid);
   tend(3);
   if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
     tbegin(4);
@@ -571,15 +600,19 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
 }
 
 template <typename T> void Batch::iteration(bool xstate) {
-  const int nb = dm.ncmap[0];
+  const int nb = dm.ncmap[0], n = (int)order.size();
   const BlkMap *m0 = dm.cmap[0];
-  tbegin(8); launch_bicg<T>(stream, 0, d_insts, m0, nb, d_cnt, xstate); tend(8);
+  tbegin(8); launch_bicg<T>(stream, 0, d_insts, m0, nb, xstate); tend(8);
   vcycle<T>(0, VPH, VP);
-  tbegin(6); launch_bicg<T>(stream, 1, d_insts, m0, nb, d_cnt, xstate); tend(6);
-  tbegin(8); launch_bicg<T>(stream, 2, d_insts, m0, nb, d_cnt, xstate); tend(8);
+  tbegin(6); launch_bicg<T>(stream, 1, d_insts, m0, nb, xstate); tend(6);
+  launch_finish<T>(stream, FIN_V, d_insts, n, d_cnt);
+  tbegin(8); launch_bicg<T>(stream, 2, d_insts, m0, nb, xstate); tend(8);
+  launch_finish<T>(stream, FIN_S, d_insts, n, d_cnt);
   vcycle<T>(0, VSH, VS);
-  tbegin(7); launch_bicg<T>(stream, 3, d_insts, m0, nb, d_cnt, xstate); tend(7);
-  tbegin(8); launch_bicg<T>(stream, 4, d_insts, m0, nb, d_cnt, xstate); tend(8);
+  tbegin(7); launch_bicg<T>(stream, 3, d_insts, m0, nb, xstate); tend(7);
+  launch_finish<T>(stream, FIN_T, d_insts, n, d_cnt);
+  tbegin(8); launch_bicg<T>(stream, 4, d_insts, m0, nb, xstate); tend(8);
+  launch_finish<T>(stream, FIN_X, d_insts, n, d_cnt);
 }
 
 template <typename T> int Batch::krylov(int maxit, bool xstate) {
@@ -617,28 +650,41 @@ int Batch::step_once() {
   const int nb0 = dm.ncmap[0];
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
   if (mixed) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
-  for (int l = 0; l + 1 < dm.maxlev; ++l) {
+  for (int l = 0; l < dm.maxlev; ++l) {
     tbegin(1);
-    launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, mixed);
-    launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed);
+    // LU factors of the line matrices: smoothing lines of both colours and the
+    // exact solve on the coarsest level
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p)
+        launch_line_factor(stream, d_insts, dm.smap[l][d][p], dm.nsmap[l][d][p], l, d, mixed);
+    launch_line_factor(stream, d_insts, dm.cxmap[l], dm.ncx[l], l, 0, mixed);
+    launch_line_factor(stream, d_insts, dm.cymap[l], dm.ncy[l], l, 1, mixed);
+    if (l + 1 < dm.maxlev) {
+      launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, mixed);
+      launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed);
+    }
     tend(1);
   }
   if (!mixed) {
-    launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0, d_cnt);
+    launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0);
+    launch_finish<double>(stream, FIN_INIT64, d_insts, n, d_cnt);
     if ((rc = krylov<double>(opts.max_iters, true))) return rc;
   } else {
     for (int ref = 0; ref <= opts.max_iters; ++ref) {
-      tbegin(9); launch_ir_residual(stream, d_insts, dm.cmap[0], nb0, d_cnt + 1, ref == 0); tend(9);
+      tbegin(9); launch_ir_residual(stream, d_insts, dm.cmap[0], nb0); tend(9);
+      launch_finish<double>(stream, FIN_IR_RES, d_insts, n, d_cnt + 1);
       int oactive = 0;
       if ((rc = read_counter(1, &oactive))) return rc;
       if (!oactive) break;
-      launch_ir_inner_init(stream, d_insts, dm.cmap[0], nb0, d_cnt);
+      launch_ir_inner_init(stream, d_insts, dm.cmap[0], nb0);
+      launch_finish<float>(stream, FIN_INNER_INIT, d_insts, n, d_cnt);
       launch_copy_counter(stream, d_cnt + 1, d_cnt);
       if ((rc = krylov<float>(opts.max_inner, false))) return rc;
       launch_ir_update(stream, d_insts, dm.cmap[0], nb0);
     }
   }
   tbegin(10); launch_finalize(stream, d_insts, dm.cmap[0], nb0); tend(10);
+  launch_finish<double>(stream, FIN_FINALIZE, d_insts, n, d_cnt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(h_scal.data(), d_scal, sizeof(KScal) * slots.size(), hipMemcpyDeviceToHost,
                         stream));

@@ -14,6 +14,8 @@ namespace sg {
 constexpr int kMaxLev = 16;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
 constexpr int kLineBlock = 64;  // lines per workgroup in line kernels (one wave)
+constexpr int kTile = 16;       // steps along the line per LDS tile
+constexpr int kLineThreads = 256;   // threads of a line workgroup (4 per tile step of a line)
 
 // band slots; level 0 uses the first five (the operator is 5-point), coarse
 // Galerkin levels use all nine.
@@ -45,6 +47,8 @@ struct Lev {            // one multigrid level of one instance
   double *band64[9];    // FP64 operator (level 0: the 5 assembled bands)
   void *band[9];        // V-cycle precision copy (aliases band64 in FP64 mode)
   void *pw[4];          // prolongation weights to the next coarser level
+  void *linv[2], *lup[2];   // LU factors of the row (0) and column (1) line
+                            // matrices: 1/pivot and the eliminated upper entry
   void *x, *b, *cp, *dp;
 };
 
@@ -72,5 +76,6 @@ struct InstDev {
 
 struct BlkMap { int inst; int k0; };      // cell-parallel: 256 cells of one instance
 struct LineMap { int inst; int line; };   // line-parallel: one row or column
+struct LineGrp { int inst; int first; int count; int step; };   // lines first + k*step, k < count
 
 }  // namespace sg

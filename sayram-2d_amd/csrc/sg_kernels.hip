@@ -22,8 +22,7 @@ template <typename T> __device__ __forceinline__ const T *cptr(const void *p) {
 }
 
 // ---------------------------------------------------------------------------
-// reductions: wave64 shuffles, LDS across the 4 waves, then a deterministic
-// per-instance "last workgroup" pass over the partial sums.
+// workgroup reduction: wave64 shuffles, then LDS across the 4 waves
 // ---------------------------------------------------------------------------
 template <typename A, int NV>
 __device__ __forceinline__ void block_sum(A (&v)[NV]) {
@@ -50,42 +49,6 @@ __device__ __forceinline__ void block_sum(A (&v)[NV]) {
     }
   }
   __syncthreads();
-}
-
-// Writes this workgroup's NV partial sums, and returns true in the last
-// workgroup of the instance to finish, with the instance totals in tot[] (in
-// thread 0). The totals are summed in a fixed order, so results do not depend
-// on scheduling or on which other instances share the launch.
-template <typename A, int NV>
-__device__ bool instance_reduce(const InstDev &I, int lblk, A (&v)[NV], A (&tot)[NV],
-                                int which_ticket) {
-  __shared__ int s_last;
-  block_sum<A, NV>(v);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) I.partial[(size_t)k * I.nblk0 + lblk] = (double)v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(&I.ticket[which_ticket], 1u);
-    s_last = (t == (unsigned)(I.nblk0 - 1));
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  A acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = A(0);
-  for (int b = threadIdx.x; b < I.nblk0; b += kBlock) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      acc[k] += (A)__builtin_nontemporal_load(&I.partial[(size_t)k * I.nblk0 + b]);
-  }
-  block_sum<A, NV>(acc);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-    I.ticket[which_ticket] = 0u;   // reusable by the next launch without a memset
-  }
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -399,7 +362,9 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
 }
 
 // A_{l+1}(C, C') = sum_f sum_g P(f, C) A_l(f, g) P(g, C'); one work-item per
-// coarse point, whose support is the 3x3 fine block around (2I, 2J).
+// coarse point C = (I, J), whose support is the 3x3 fine block around (2I, 2J).
+// All offsets are compile-time constants (fine offset fa, fb; stencil offset;
+// parent slot q), so the 3x3 accumulator lives in registers.
 template <typename T, int NB, bool COPY>
 __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ insts,
                                                const BlkMap *__restrict__ map, int l) {
@@ -412,43 +377,37 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   const int KC = m.k0 + threadIdx.x;
   if (KC >= cx * cy) return;
   const int J = KC / cx, Ic = KC - J * cx;
-  const T *__restrict__ pw0 = cptr<T>(F.pw[0]);
-  const T *__restrict__ pw1 = cptr<T>(F.pw[1]);
-  const T *__restrict__ pw2 = cptr<T>(F.pw[2]);
-  const T *__restrict__ pw3 = cptr<T>(F.pw[3]);
-  double acc[3][3];
+  const T *__restrict__ pw[4] = {cptr<T>(F.pw[0]), cptr<T>(F.pw[1]), cptr<T>(F.pw[2]),
+                                 cptr<T>(F.pw[3])};
+  constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
+  constexpr int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
+  double acc[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc[a][c] = 0.0;
-  const int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
-  const int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
-  for (int fj = 2 * J - 1; fj <= 2 * J + 1; ++fj) {
+  for (int fb = -1; fb <= 1; ++fb) {
+    const int fj = 2 * J + fb;
     if (fj < 0 || fj >= fy) continue;
-    const int dJ = J - (fj >> 1);
-    for (int fi = 2 * Ic - 1; fi <= 2 * Ic + 1; ++fi) {
+#pragma unroll
+    for (int fa = -1; fa <= 1; ++fa) {
+      const int fi = 2 * Ic + fa;
       if (fi < 0 || fi >= fx) continue;
-      const int dI = Ic - (fi >> 1);
       const size_t kf = (size_t)fj * fx + fi;
-      const int slot = dI + 2 * dJ;
-      const double wf = (double)(slot == 0 ? pw0[kf] : slot == 1 ? pw1[kf] : slot == 2 ? pw2[kf] : pw3[kf]);
-      if (wf == 0.0) continue;
+      // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
+      const int slot = (fa < 0 ? 1 : 0) + 2 * (fb < 0 ? 1 : 0);
+      const double wf = (double)pw[slot][kf];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const int gi = fi + di_[b], gj = fj + dj_[b];
+        const int ga = fa + di_[b], gb = fb + dj_[b];
+        const int gi = 2 * Ic + ga, gj = 2 * J + gb;
         if (gi < 0 || gi >= fx || gj < 0 || gj >= fy) continue;
-        const double av = F.band64[b][kf];
-        if (av == 0.0) continue;
-        const double wa = wf * av;
+        const double wa = wf * F.band64[b][kf];
         const size_t kg = (size_t)gj * fx + gi;
-        const int oi = (gi >> 1) - Ic + 1, oj = (gj >> 1) - J + 1;
-        const double wg[4] = {(double)pw0[kg], (double)pw1[kg], (double)pw2[kg], (double)pw3[kg]};
+        const int oi = (ga >= 0 ? ga / 2 : -((1 - ga) / 2)) + 1;   // floor(ga / 2) + 1
+        const int oj = (gb >= 0 ? gb / 2 : -((1 - gb) / 2)) + 1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (wg[q] == 0.0) continue;
           const int ai = oi + (q & 1), aj = oj + (q >> 1);
-          if (ai < 0 || ai > 2 || aj < 0 || aj > 2) continue;
-          acc[aj][ai] += wa * wg[q];
+          if (ai > 2 || aj > 2) continue;   // such parents carry zero weight
+          acc[aj][ai] += wa * (double)pw[q][kg];
         }
       }
     }
@@ -472,95 +431,185 @@ __device__ __forceinline__ void level_xb(const InstDev &I, int l, int xid, int b
   else { x = ptr<T>(I.lev[l].x); b = cptr<T>(I.lev[l].b); }
 }
 
-// Zebra line Gauss-Seidel: every lane solves one row (DIR 0) or column (DIR 1)
-// exactly with the Thomas algorithm, with the off-line couplings moved to the
-// right-hand side. ZERO: the other lines are zero (first sweep of a cycle).
-template <typename T, int NB, int DIR, bool ZERO>
-__global__ void __launch_bounds__(kLineBlock) k_line(const InstDev *__restrict__ insts,
-                                                    const LineMap *__restrict__ map, int nlines,
-                                                    int l, int xid, int bid) {
+// LU factors of every line matrix of a level (rows: DIR 0, columns: DIR 1),
+// once per step: lup_t = upper_t / pivot_t, linv_t = 1 / pivot_t with
+// pivot_t = diag_t - lower_t lup_{t-1} (Thomas elimination without pivoting;
+// the level-0 line matrices are column diagonally dominant M-matrices).
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kLineBlock) k_line_factor(const InstDev *__restrict__ insts,
+                                                           const LineMap *__restrict__ map,
+                                                           int nlines, int l) {
   const int g = blockIdx.x * kLineBlock + threadIdx.x;
   if (g >= nlines) return;
   const LineMap m = map[g];
   const InstDev &I = insts[m.inst];
+  if (I.sc->ostatus == ST_IDLE) return;
+  const Lev &L = I.lev[l];
+  const int nx = L.nx, ny = L.ny;
+  const int n = DIR == 0 ? nx : ny;
+  const size_t k0 = DIR == 0 ? (size_t)m.line * nx : (size_t)m.line;
+  const size_t st = DIR == 0 ? 1 : (size_t)nx;
+  const double *__restrict__ bc = L.band64[BC];
+  const double *__restrict__ lo = L.band64[DIR == 0 ? BW : BS];
+  const double *__restrict__ up = L.band64[DIR == 0 ? BE : BN];
+  T *__restrict__ inv = ptr<T>(L.linv[DIR]);
+  T *__restrict__ cu = ptr<T>(L.lup[DIR]);
+  double cprev = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const size_t K = k0 + (size_t)t * st;
+    const double a = t > 0 ? lo[K] : 0.0;
+    const double iv = 1.0 / (bc[K] - a * cprev);
+    cprev = t < n - 1 ? up[K] * iv : 0.0;
+    inv[K] = (T)iv;
+    cu[K] = (T)cprev;
+  }
+}
+
+// Zebra line Gauss-Seidel. A workgroup solves a group of up to 64 lines of one
+// colour exactly with the precomputed factors, the off-line couplings moved to
+// the right-hand side. The lines are walked in tiles of kTile steps staged
+// through LDS: the 256 threads load a 64 x kTile tile with coalesced global
+// accesses (along the line for rows, across the lines for columns), every
+// thread issuing the loads of its 4 elements together, then the first wave runs
+// the 64 recurrences on the tile from LDS (one lane per line). The forward pass
+// stores the eliminated right-hand side in x (no other line of this colour
+// reads it), the backward pass turns it into the solution. ZERO: the other
+// lines are zero (first sweep of a cycle).
+template <typename T, int NB, int DIR, bool ZERO>
+__global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict__ insts,
+                                                      const LineGrp *__restrict__ map, int l,
+                                                      int xid, int bid) {
+  constexpr int Q = kLineBlock * kTile / kLineThreads;   // tile elements per thread
+  __shared__ T s_r[kLineBlock][kTile + 1];
+  __shared__ T s_a[kLineBlock][kTile + 1];
+  __shared__ T s_i[kLineBlock][kTile + 1];
+  const LineGrp gr = map[blockIdx.x];
+  const InstDev &I = insts[gr.inst];
   if (I.sc->status != ST_ACTIVE) return;
   const Lev &L = I.lev[l];
   const int nx = L.nx, ny = L.ny;
   T *x; const T *b;
   level_xb<T>(I, l, xid, bid, x, b);
-  const T *__restrict__ bc = cptr<T>(L.band[BC]);
-  const T *__restrict__ bw = cptr<T>(L.band[BW]);
-  const T *__restrict__ be = cptr<T>(L.band[BE]);
-  const T *__restrict__ bs = cptr<T>(L.band[BS]);
-  const T *__restrict__ bn = cptr<T>(L.band[BN]);
-  const T *bsw = nullptr, *bse = nullptr, *bnw = nullptr, *bne = nullptr;
-  if (NB == 9) {
-    bsw = cptr<T>(L.band[BSW]); bse = cptr<T>(L.band[BSE]);
-    bnw = cptr<T>(L.band[BNW]); bne = cptr<T>(L.band[BNE]);
+  const T *__restrict__ lo = cptr<T>(L.band[DIR == 0 ? BW : BS]);
+  const T *__restrict__ om = cptr<T>(L.band[DIR == 0 ? BS : BW]);
+  const T *__restrict__ op = cptr<T>(L.band[DIR == 0 ? BN : BE]);
+  const T *omm = nullptr, *omp = nullptr, *opm = nullptr, *opp = nullptr;
+  if (NB == 9) {   // diagonal couplings: (minus side, along -1/+1), (plus side, along -1/+1)
+    omm = cptr<T>(L.band[BSW]);
+    omp = cptr<T>(L.band[DIR == 0 ? BSE : BNW]);
+    opm = cptr<T>(L.band[DIR == 0 ? BNW : BSE]);
+    opp = cptr<T>(L.band[BNE]);
   }
-  T *__restrict__ cp = ptr<T>(L.cp);
-  T *__restrict__ dp = ptr<T>(L.dp);
-  const int n = DIR == 0 ? nx : ny;
-  const size_t k0 = DIR == 0 ? (size_t)m.line * nx : (size_t)m.line;
-  const size_t st = DIR == 0 ? 1 : (size_t)nx;
-  const int li = m.line;   // row j (DIR 0) or column i (DIR 1)
-  T cprev = T(0), dprev = T(0);
-  for (int t = 0; t < n; ++t) {
-    const size_t K = k0 + (size_t)t * st;
-    const int i = DIR == 0 ? t : li, j = DIR == 0 ? li : t;
-    T rhs = b[K];
-    T a, c;
-    if (DIR == 0) {
-      a = i > 0 ? bw[K] : T(0);
-      c = i < nx - 1 ? be[K] : T(0);
-      if (!ZERO) {
-        if (j > 0) {
-          rhs -= bs[K] * x[K - nx];
-          if (NB == 9) {
-            if (i > 0) rhs -= bsw[K] * x[K - nx - 1];
-            if (i < nx - 1) rhs -= bse[K] * x[K - nx + 1];
+  const T *__restrict__ inv = cptr<T>(L.linv[DIR]);
+  const T *__restrict__ cu = cptr<T>(L.lup[DIR]);
+  const int n = DIR == 0 ? nx : ny;            // line length
+  const int nperp = DIR == 0 ? ny : nx;
+  const size_t st = DIR == 0 ? 1 : (size_t)nx;  // along the line
+  const size_t so = DIR == 0 ? (size_t)nx : 1;  // to the neighbouring lines
+  const int cnt = gr.count;
+  const int tid = threadIdx.x;
+  // element e of a tile -> (line k, step u), in coalesced order in memory
+  int ek[Q], eu[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = q * kLineThreads + tid;
+    ek[q] = DIR == 0 ? e / kTile : e % kLineBlock;
+    eu[q] = DIR == 0 ? e % kTile : e / kLineBlock;
+  }
+  T dprev = T(0);
+  for (int t0 = 0; t0 < n; t0 += kTile) {
+    const int tw = min(kTile, n - t0);
+    T vb[Q], vlo[Q], viv[Q], vom[Q], vop[Q], xm[Q], xp[Q];
+    T dmm[Q], dmp[Q], dpm[Q], dpp[Q], xmm[Q], xmp[Q], xpm[Q], xpp[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {   // issue every load of the tile first
+      const int k = ek[q], u = eu[q];
+      vb[q] = vlo[q] = viv[q] = vom[q] = vop[q] = xm[q] = xp[q] = T(0);
+      dmm[q] = dmp[q] = dpm[q] = dpp[q] = xmm[q] = xmp[q] = xpm[q] = xpp[q] = T(0);
+      if (k < cnt && u < tw) {
+        const int li = gr.first + k * gr.step, t = t0 + u;
+        const size_t K = (size_t)li * so + (size_t)t * st;
+        vb[q] = b[K];
+        vlo[q] = t > 0 ? lo[K] : T(0);
+        viv[q] = inv[K];
+        if (!ZERO) {
+          if (li > 0) {
+            vom[q] = om[K]; xm[q] = x[K - so];
+            if (NB == 9) {
+              if (t > 0) { dmm[q] = omm[K]; xmm[q] = x[K - so - st]; }
+              if (t < n - 1) { dmp[q] = omp[K]; xmp[q] = x[K - so + st]; }
+            }
           }
-        }
-        if (j < ny - 1) {
-          rhs -= bn[K] * x[K + nx];
-          if (NB == 9) {
-            if (i > 0) rhs -= bnw[K] * x[K + nx - 1];
-            if (i < nx - 1) rhs -= bne[K] * x[K + nx + 1];
-          }
-        }
-      }
-    } else {
-      a = j > 0 ? bs[K] : T(0);
-      c = j < ny - 1 ? bn[K] : T(0);
-      if (!ZERO) {
-        if (i > 0) {
-          rhs -= bw[K] * x[K - 1];
-          if (NB == 9) {
-            if (j > 0) rhs -= bsw[K] * x[K - nx - 1];
-            if (j < ny - 1) rhs -= bnw[K] * x[K + nx - 1];
-          }
-        }
-        if (i < nx - 1) {
-          rhs -= be[K] * x[K + 1];
-          if (NB == 9) {
-            if (j > 0) rhs -= bse[K] * x[K - nx + 1];
-            if (j < ny - 1) rhs -= bne[K] * x[K + nx + 1];
+          if (li < nperp - 1) {
+            vop[q] = op[K]; xp[q] = x[K + so];
+            if (NB == 9) {
+              if (t > 0) { dpm[q] = opm[K]; xpm[q] = x[K + so - st]; }
+              if (t < n - 1) { dpp[q] = opp[K]; xpp[q] = x[K + so + st]; }
+            }
           }
         }
       }
     }
-    const T inv = T(1) / (bc[K] - a * cprev);
-    cprev = c * inv;
-    dprev = (rhs - a * dprev) * inv;
-    cp[K] = cprev;
-    dp[K] = dprev;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      T r = vb[q] - vom[q] * xm[q] - vop[q] * xp[q];
+      if (NB == 9) r = r - dmm[q] * xmm[q] - dmp[q] * xmp[q] - dpm[q] * xpm[q] - dpp[q] * xpp[q];
+      s_r[ek[q]][eu[q]] = r;
+      s_a[ek[q]][eu[q]] = vlo[q];
+      s_i[ek[q]][eu[q]] = viv[q];
+    }
+    __syncthreads();
+    if (tid < cnt) {
+      for (int u = 0; u < tw; ++u) {
+        dprev = (s_r[tid][u] - s_a[tid][u] * dprev) * s_i[tid][u];
+        s_r[tid][u] = dprev;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = ek[q], u = eu[q];
+      if (k < cnt && u < tw)
+        x[(size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st] = s_r[k][u];
+    }
+    __syncthreads();
   }
-  T xn = dprev;
-  x[k0 + (size_t)(n - 1) * st] = xn;
-  for (int t = n - 2; t >= 0; --t) {
-    const size_t K = k0 + (size_t)t * st;
-    xn = dp[K] - cp[K] * xn;
-    x[K] = xn;
+  T xn = T(0);
+  const int ntiles = (n + kTile - 1) / kTile;
+  for (int tt = ntiles - 1; tt >= 0; --tt) {
+    const int t0 = tt * kTile;
+    const int tw = min(kTile, n - t0);
+    T vd[Q], vc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = ek[q], u = eu[q];
+      vd[q] = vc[q] = T(0);
+      if (k < cnt && u < tw) {
+        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st;
+        vd[q] = x[K];
+        vc[q] = cu[K];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      s_r[ek[q]][eu[q]] = vd[q];
+      s_a[ek[q]][eu[q]] = vc[q];
+    }
+    __syncthreads();
+    if (tid < cnt) {
+      for (int u = tw - 1; u >= 0; --u) {
+        xn = s_r[tid][u] - s_a[tid][u] * xn;
+        s_r[tid][u] = xn;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = ek[q], u = eu[q];
+      if (k < cnt && u < tw)
+        x[(size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st] = s_r[k][u];
+    }
+    __syncthreads();
   }
 }
 
@@ -660,19 +709,33 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
 // ---------------------------------------------------------------------------
 // BiCGSTAB (right-preconditioned), precision T for vectors and sums; the
 // operator is the level-0 5-band matrix of the V-cycle precision.
+//
+// Reductions are two-phase: every workgroup stores its partial sums (plain
+// stores), and a finish kernel with one workgroup per instance adds them in a
+// fixed order and updates the scalars. The kernel boundary orders the two, so
+// no device-wide fence or atomic is needed, and the result of an instance does
+// not depend on the rest of the batch.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T spmv5(const InstDev &I, size_t K, int i, int j, const T *x) {
   return stencil_apply<T, 5>(I.lev[0], K, i, j, x);
 }
 
-// FP64 start: r = R - M x, rh = r, p = v = 0; rho = (r, r), bnorm = ||R||.
+template <typename A, int NV>
+__device__ __forceinline__ void store_partials(const InstDev &I, int lblk, A (&v)[NV]) {
+  block_sum<A, NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) I.partial[(size_t)k * I.nblk0 + lblk] = (double)v[k];
+  }
+}
+
+// FP64 start: r = R - M x, rh = r, p = v = 0; partials (r, r), (R, R).
 __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restrict__ insts,
-                                                       const BlkMap *__restrict__ map,
-                                                       int *__restrict__ active) {
+                                                       const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (I.sc->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   double v[2] = {0.0, 0.0};
@@ -686,15 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
     v[0] = r * r;
     v[1] = I.R[K] * I.R[K];
   }
-  double tot[2];
-  if (instance_reduce<double, 2>(I, m.k0 / kBlock, v, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    s.rho = tot[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
-    s.rnorm2 = tot[0]; s.bnorm2 = tot[1]; s.iters = 0;
-    const bool done = tot[0] <= s.tol2 * tot[1];
-    s.status = done ? ST_DONE : ST_ACTIVE;
-    if (done) atomicSub(active, 1);
-  }
+  store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
 
 // p = r + beta (p - omega v)
@@ -711,7 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
   p[K] = cptr<T>(I.kv[VR])[K] + beta * (p[K] - omega * cptr<T>(I.kv[VV])[K]);
 }
 
-// v = M ph; alpha = rho / (rh, v)
+// v = M ph; partial (rh, v)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
@@ -727,14 +782,10 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
     ptr<T>(I.kv[VV])[K] = v;
     acc[0] = cptr<T>(I.kv[VRH])[K] * v;
   }
-  T tot[1];
-  if (instance_reduce<T, 1>(I, m.k0 / kBlock, acc, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    s.alpha = (double)((T)s.rho / tot[0]);
-  }
+  store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
 
-// s = r - alpha v; if ||s|| is small the step ends at x + alpha ph (HALF)
+// s = r - alpha v; partial (s, s)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
@@ -749,14 +800,10 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ i
     ptr<T>(I.kv[VS])[K] = s;
     acc[0] = s * s;
   }
-  T tot[1];
-  if (instance_reduce<T, 1>(I, m.k0 / kBlock, acc, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    if ((double)tot[0] <= s.tol2 * s.bnorm2) { s.status = ST_HALF; s.rnorm2 = (double)tot[0]; }
-  }
+  store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
 
-// t = M sh; omega = (t, s) / (t, t)
+// t = M sh; partials (t, s), (t, t)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
@@ -773,20 +820,15 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
     acc[0] = t * cptr<T>(I.kv[VS])[K];
     acc[1] = t * t;
   }
-  T tot[2];
-  if (instance_reduce<T, 2>(I, m.k0 / kBlock, acc, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    s.omega = tot[1] > T(0) ? (double)(tot[0] / tot[1]) : 0.0;
-  }
+  store_partials<T, 2>(I, m.k0 / kBlock, acc);
 }
 
-// x += alpha ph + omega sh; r = s - omega t; rho = (rh, r); convergence.
-// XV: slot of the solution vector (the FP64 state x, or the FP32 inner
-// correction in mixed precision).
+// x += alpha ph + omega sh; r = s - omega t; partials (rh, r), (r, r).
+// HALF (converged at s): x += alpha ph only. XSTATE: x is the FP64 state,
+// otherwise the inner FP32 correction of mixed precision.
 template <typename T, bool XSTATE>
 __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ insts,
-                                                  const BlkMap *__restrict__ map,
-                                                  int *__restrict__ active) {
+                                                  const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   const int st = I.sc->status;
@@ -807,37 +849,12 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ i
       acc[1] = r * r;
     }
   }
-  T tot[2];
-  if (instance_reduce<T, 2>(I, m.k0 / kBlock, acc, tot, 1) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    s.iters += 1;
-    s.titers += 1;
-    if (st == ST_HALF) {
-      s.status = ST_DONE;
-      atomicSub(active, 1);
-      return;
-    }
-    s.rho_old = s.rho;
-    s.rho = (double)tot[0];
-    s.rnorm2 = (double)tot[1];
-    s.beta = (double)(((T)s.rho / (T)s.rho_old) * (alpha / omega));
-    if (s.rnorm2 <= s.tol2 * s.bnorm2) {
-      s.status = ST_DONE;
-      atomicSub(active, 1);
-    } else if (s.iters >= s.maxit || !(s.rnorm2 == s.rnorm2) || s.rho == 0.0 || s.omega == 0.0) {
-      s.status = ST_FAIL;
-      atomicSub(active, 1);
-    }
-  }
+  if (st == ST_ACTIVE) store_partials<T, 2>(I, m.k0 / kBlock, acc);
 }
 
-// ---------------------------------------------------------------------------
-// mixed precision: FP64 residual and refinement around an FP32 solve
-// ---------------------------------------------------------------------------
-// r64 = R - M x (FP64 bands); decides whether another refinement is needed.
+// mixed precision: r64 = R - M x with the FP64 bands; partials (r, r), (R, R)
 __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restrict__ insts,
-                                                       const BlkMap *__restrict__ map,
-                                                       int *__restrict__ oactive, int first) {
+                                                       const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (I.sc->ostatus != ST_ACTIVE) return;
@@ -846,47 +863,22 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restric
   double v[2] = {0.0, 0.0};
   if (K < N) {
     const int j = K / I.nx, i = K - j * I.nx;
-    const Lev &L = I.lev[0];
     double s = I.band64[BC][K] * I.x[K];
     if (i > 0) s += I.band64[BW][K] * I.x[K - 1];
-    if (i < L.nx - 1) s += I.band64[BE][K] * I.x[K + 1];
-    if (j > 0) s += I.band64[BS][K] * I.x[K - L.nx];
-    if (j < L.ny - 1) s += I.band64[BN][K] * I.x[K + L.nx];
+    if (i < I.nx - 1) s += I.band64[BE][K] * I.x[K + 1];
+    if (j > 0) s += I.band64[BS][K] * I.x[K - I.nx];
+    if (j < I.ny - 1) s += I.band64[BN][K] * I.x[K + I.nx];
     const double r = I.R[K] - s;
     I.r64[K] = r;
     v[0] = r * r;
-    v[1] = first ? I.R[K] * I.R[K] : 0.0;
+    v[1] = I.R[K] * I.R[K];
   }
-  double tot[2];
-  if (instance_reduce<double, 2>(I, m.k0 / kBlock, v, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    if (first) s.obnorm2 = tot[1];
-    s.onorm2 = tot[0];
-    s.scale = sqrt(tot[0]);
-    // Converged; or stalled at the FP64 rounding floor of R - M x (the
-    // residual stopped halving within 100x of the tolerance).
-    const bool conv = tot[0] <= s.otol2 * s.obnorm2;
-    const bool floor = !first && tot[0] > 0.25 * s.oprev2 && tot[0] <= 1e4 * s.otol2 * s.obnorm2;
-    s.oprev2 = tot[0];
-    if (conv || floor) {
-      s.ostatus = ST_DONE;
-      s.status = ST_DONE;
-      atomicSub(oactive, 1);
-    } else if (s.orefs >= s.omaxit) {
-      s.ostatus = ST_FAIL;
-      s.status = ST_DONE;
-      atomicSub(oactive, 1);
-    } else {
-      s.orefs += 1;
-      s.status = ST_ACTIVE;
-    }
-  }
+  store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
 
-// inner start: r = r64 / ||r64|| (FP32), rh = r, p = v = xk = 0
+// inner start: r = r64 / ||r64|| (FP32), rh = r, p = v = xk = 0; partial (r, r)
 __global__ void __launch_bounds__(kBlock) k_ir_inner_init(const InstDev *__restrict__ insts,
-                                                         const BlkMap *__restrict__ map,
-                                                         int *__restrict__ active) {
+                                                         const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (I.sc->ostatus != ST_ACTIVE) return;
@@ -902,14 +894,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_inner_init(const InstDev *__restr
     ptr<float>(I.kv[VXK])[K] = 0.0f;
     v[0] = r * r;
   }
-  float tot[1];
-  if (instance_reduce<float, 1>(I, m.k0 / kBlock, v, tot, 0) && threadIdx.x == 0) {
-    KScal &s = *I.sc;
-    s.rho = tot[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
-    s.rnorm2 = tot[0]; s.bnorm2 = tot[0]; s.iters = 0;
-    s.status = ST_ACTIVE;
-    (void)active;
-  }
+  store_partials<float, 1>(I, m.k0 / kBlock, v);
 }
 
 // x += ||r64|| * xk (FP64)
@@ -923,7 +908,8 @@ __global__ void __launch_bounds__(kBlock) k_ir_update(const InstDev *__restrict_
   I.x[K] += I.sc->scale * (double)ptr<float>(I.kv[VXK])[K];
 }
 
-// f = max(x, 0); true residual of the unclamped x; min f; clamp count.
+// f = max(x, 0); partials of the true residual of the unclamped x, ||R||, the
+// clamp count and the minimum (slot 3).
 __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -931,8 +917,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
   if (I.sc->ostatus == ST_IDLE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};   // r^2, R^2, clamps, -min
-  __shared__ double smin[kBlock];
+  double v[3] = {0.0, 0.0, 0.0};
   double mn = DBL_MAX;
   if (K < N) {
     const int j = K / I.nx, i = K - j * I.nx;
@@ -948,39 +933,156 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
     v[2] = xv < 0.0 ? 1.0 : 0.0;
     mn = xv;
   }
-  smin[threadIdx.x] = mn;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + o]);
-    __syncthreads();
-  }
-  v[3] = 0.0;
-  // min is carried through the partial array as a separate slot by the last block
-  double tot[3];
-  double v3[3] = {v[0], v[1], v[2]};
-  const double bmin = smin[0];
-  if (threadIdx.x == 0) I.partial[(size_t)3 * I.nblk0 + m.k0 / kBlock] = bmin;
-  if (instance_reduce<double, 3>(I, m.k0 / kBlock, v3, tot, 0)) {
-    __shared__ double sm2[kBlock];
-    double mm = DBL_MAX;
-    for (int b = threadIdx.x; b < I.nblk0; b += kBlock)
-      mm = fmin(mm, __builtin_nontemporal_load(&I.partial[(size_t)3 * I.nblk0 + b]));
-    sm2[threadIdx.x] = mm;
-    __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) sm2[threadIdx.x] = fmin(sm2[threadIdx.x], sm2[threadIdx.x + o]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      KScal &s = *I.sc;
-      s.true_rel = tot[1] > 0.0 ? sqrt(tot[0] / tot[1]) : sqrt(tot[0]);
-      s.clamps = (int)tot[2];
-      s.minf = sm2[0];
-    }
+  __shared__ double smin[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mn = fmin(mn, __shfl_down(mn, off, 64));
+  if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = mn;
+  store_partials<double, 3>(I, m.k0 / kBlock, v);   // contains __syncthreads
+  if (threadIdx.x == 0) {
+    double b = smin[0];
+    for (int w = 1; w < kBlock / 64; ++w) b = fmin(b, smin[w]);
+    I.partial[(size_t)3 * I.nblk0 + m.k0 / kBlock] = b;
   }
   if (K < N) {
     const double xv = I.x[K];
     I.f[K] = xv > 0.0 ? xv : 0.0;   // min f >= 0 enforced exactly
+  }
+}
+
+// Finish of a reduction: one workgroup per instance sums the partial sums in a
+// fixed order and applies the BiCGSTAB / refinement scalar updates.
+enum Finish { F_INIT64 = 0, F_V, F_S, F_T, F_X, F_IR_RES, F_INNER_INIT, F_FINALIZE };
+
+template <typename A, int NV>
+__device__ __forceinline__ void sum_partials(const InstDev &I, A (&tot)[NV]) {
+  A acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = A(0);
+  for (int b = threadIdx.x; b < I.nblk0; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += (A)I.partial[(size_t)k * I.nblk0 + b];
+  }
+  block_sum<A, NV>(acc);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = acc[k];   // valid in thread 0
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ insts, int n,
+                                                  int *__restrict__ cnt) {
+  const int k = blockIdx.x;
+  if (k >= n) return;
+  const InstDev &I = insts[k];
+  KScal &s = *I.sc;
+  const int st = s.status, ost = s.ostatus;
+  if (MODE == F_IR_RES || MODE == F_INNER_INIT) {
+    if (ost != ST_ACTIVE) return;
+  } else if (MODE == F_FINALIZE) {
+    if (ost == ST_IDLE) return;
+  } else if (MODE == F_X) {
+    if (st != ST_ACTIVE && st != ST_HALF) return;
+  } else {
+    if (st != ST_ACTIVE) return;
+  }
+  if (MODE == F_INIT64) {
+    double t[2];
+    sum_partials<double, 2>(I, t);
+    if (threadIdx.x == 0) {
+      s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
+      s.rnorm2 = t[0]; s.bnorm2 = t[1]; s.iters = 0;
+      if (t[0] <= s.tol2 * t[1]) { s.status = ST_DONE; atomicSub(cnt, 1); }
+    }
+  } else if (MODE == F_V) {
+    T t[1];
+    sum_partials<T, 1>(I, t);
+    if (threadIdx.x == 0) s.alpha = (double)((T)s.rho / t[0]);
+  } else if (MODE == F_S) {
+    T t[1];
+    sum_partials<T, 1>(I, t);
+    if (threadIdx.x == 0 && (double)t[0] <= s.tol2 * s.bnorm2) {
+      s.status = ST_HALF;
+      s.rnorm2 = (double)t[0];
+    }
+  } else if (MODE == F_T) {
+    T t[2];
+    sum_partials<T, 2>(I, t);
+    if (threadIdx.x == 0) s.omega = t[1] > T(0) ? (double)(t[0] / t[1]) : 0.0;
+  } else if (MODE == F_X) {
+    T t[2] = {T(0), T(0)};
+    if (st == ST_ACTIVE) sum_partials<T, 2>(I, t);
+    if (threadIdx.x == 0) {
+      s.iters += 1;
+      s.titers += 1;
+      if (st == ST_HALF) {
+        s.status = ST_DONE;
+        atomicSub(cnt, 1);
+      } else {
+        s.rho_old = s.rho;
+        s.rho = (double)t[0];
+        s.rnorm2 = (double)t[1];
+        s.beta = (double)(((T)s.rho / (T)s.rho_old) * ((T)s.alpha / (T)s.omega));
+        if (s.rnorm2 <= s.tol2 * s.bnorm2) {
+          s.status = ST_DONE;
+          atomicSub(cnt, 1);
+        } else if (s.iters >= s.maxit || !(s.rnorm2 == s.rnorm2) || s.rho == 0.0 ||
+                   s.omega == 0.0) {
+          s.status = ST_FAIL;
+          atomicSub(cnt, 1);
+        }
+      }
+    }
+  } else if (MODE == F_IR_RES) {
+    double t[2];
+    sum_partials<double, 2>(I, t);
+    if (threadIdx.x == 0) {
+      const bool first = s.orefs == 0;
+      if (first) s.obnorm2 = t[1];
+      s.onorm2 = t[0];
+      s.scale = sqrt(t[0]);
+      // Converged; or stalled at the FP64 rounding floor of R - M x (the
+      // residual stopped halving within 100x of the tolerance).
+      const bool conv = t[0] <= s.otol2 * s.obnorm2;
+      const bool floor = !first && t[0] > 0.25 * s.oprev2 && t[0] <= 1e4 * s.otol2 * s.obnorm2;
+      s.oprev2 = t[0];
+      if (conv || floor) {
+        s.ostatus = ST_DONE;
+        s.status = ST_DONE;
+        atomicSub(cnt, 1);
+      } else if (s.orefs >= s.omaxit) {
+        s.ostatus = ST_FAIL;
+        s.status = ST_DONE;
+        atomicSub(cnt, 1);
+      } else {
+        s.orefs += 1;
+        s.status = ST_ACTIVE;
+      }
+    }
+  } else if (MODE == F_INNER_INIT) {
+    float t[1];
+    sum_partials<float, 1>(I, t);
+    if (threadIdx.x == 0) {
+      s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
+      s.rnorm2 = t[0]; s.bnorm2 = t[0]; s.iters = 0;
+      s.status = ST_ACTIVE;
+    }
+  } else {   // F_FINALIZE
+    double t[3];
+    sum_partials<double, 3>(I, t);
+    __shared__ double sm[kBlock];
+    double mm = DBL_MAX;
+    for (int b = threadIdx.x; b < I.nblk0; b += kBlock)
+      mm = fmin(mm, I.partial[(size_t)3 * I.nblk0 + b]);
+    sm[threadIdx.x] = mm;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      s.true_rel = t[1] > 0.0 ? sqrt(t[0] / t[1]) : sqrt(t[0]);
+      s.clamps = (int)t[2];
+      s.minf = sm[0];
+    }
   }
 }
 
@@ -1039,13 +1141,25 @@ void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l,
   }
 }
 
-template <typename T>
-void launch_line(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
-                 bool zero, int xid, int bid) {
+void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
+                        bool mixed) {
   if (!n) return;
   const unsigned g = nblocks(n, kLineBlock);
+  if (mixed) {
+    if (dir == 0) k_line_factor<float, 0><<<g, kLineBlock, 0, s>>>(d, m, n, l);
+    else k_line_factor<float, 1><<<g, kLineBlock, 0, s>>>(d, m, n, l);
+  } else {
+    if (dir == 0) k_line_factor<double, 0><<<g, kLineBlock, 0, s>>>(d, m, n, l);
+    else k_line_factor<double, 1><<<g, kLineBlock, 0, s>>>(d, m, n, l);
+  }
+}
+
+template <typename T>
+void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
+                 bool zero, int xid, int bid) {
+  if (!ngroups) return;
   const bool nine = l > 0;
-#define SG_LINE(NB, DIR, Z) k_line<T, NB, DIR, Z><<<g, kLineBlock, 0, s>>>(d, m, n, l, xid, bid)
+#define SG_LINE(NB, DIR, Z) k_line<T, NB, DIR, Z><<<ngroups, kLineThreads, 0, s>>>(d, m, l, xid, bid)
   if (nine) {
     if (dir == 0) { if (zero) SG_LINE(9, 0, true); else SG_LINE(9, 0, false); }
     else { if (zero) SG_LINE(9, 1, true); else SG_LINE(9, 1, false); }
@@ -1055,9 +1169,9 @@ void launch_line(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l
   }
 #undef SG_LINE
 }
-template void launch_line<double>(hipStream_t, const InstDev *, const LineMap *, int, int, int,
+template void launch_line<double>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
                                   bool, int, int);
-template void launch_line<float>(hipStream_t, const InstDev *, const LineMap *, int, int, int,
+template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
                                  bool, int, int);
 
 template <typename T>
@@ -1096,12 +1210,11 @@ template void launch_prolong<double>(hipStream_t, const InstDev *, const BlkMap 
 template void launch_prolong<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
                                     int);
 
-void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active) {
-  if (nb) k_bicg_init64<<<nb, kBlock, 0, s>>>(d, m, active);
+void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
+  if (nb) k_bicg_init64<<<nb, kBlock, 0, s>>>(d, m);
 }
 template <typename T>
-void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb,
-                 int *active, bool xstate) {
+void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate) {
   if (!nb) return;
   switch (which) {
     case 0: k_bicg_p<T><<<nb, kBlock, 0, s>>>(d, m); break;
@@ -1109,21 +1222,35 @@ void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, in
     case 2: k_bicg_s<T><<<nb, kBlock, 0, s>>>(d, m); break;
     case 3: k_bicg_t<T><<<nb, kBlock, 0, s>>>(d, m); break;
     default:
-      if (xstate) k_bicg_x<T, true><<<nb, kBlock, 0, s>>>(d, m, active);
-      else k_bicg_x<T, false><<<nb, kBlock, 0, s>>>(d, m, active);
+      if (xstate) k_bicg_x<T, true><<<nb, kBlock, 0, s>>>(d, m);
+      else k_bicg_x<T, false><<<nb, kBlock, 0, s>>>(d, m);
   }
 }
-template void launch_bicg<double>(hipStream_t, int, const InstDev *, const BlkMap *, int, int *,
-                                  bool);
-template void launch_bicg<float>(hipStream_t, int, const InstDev *, const BlkMap *, int, int *,
-                                 bool);
+template void launch_bicg<double>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool);
+template void launch_bicg<float>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool);
 
-void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *oactive,
-                        int first) {
-  if (nb) k_ir_residual<<<nb, kBlock, 0, s>>>(d, m, oactive, first);
+template <typename T>
+void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt) {
+  if (!n) return;
+  switch (mode) {
+    case F_INIT64: k_finish<T, F_INIT64><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_V: k_finish<T, F_V><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_S: k_finish<T, F_S><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_T: k_finish<T, F_T><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_X: k_finish<T, F_X><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_IR_RES: k_finish<T, F_IR_RES><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_INNER_INIT: k_finish<T, F_INNER_INIT><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    default: k_finish<T, F_FINALIZE><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+  }
 }
-void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active) {
-  if (nb) k_ir_inner_init<<<nb, kBlock, 0, s>>>(d, m, active);
+template void launch_finish<double>(hipStream_t, int, const InstDev *, int, int *);
+template void launch_finish<float>(hipStream_t, int, const InstDev *, int, int *);
+
+void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
+  if (nb) k_ir_residual<<<nb, kBlock, 0, s>>>(d, m);
+}
+void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
+  if (nb) k_ir_inner_init<<<nb, kBlock, 0, s>>>(d, m);
 }
 void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_ir_update<<<nb, kBlock, 0, s>>>(d, m);

@@ -10,8 +10,10 @@ void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
+void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
+                        bool mixed);
 template <typename T>
-void launch_line(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
+void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  bool zero, int xid, int bid);
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
@@ -21,14 +23,16 @@ void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, i
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid);
-void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active);
+void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 // which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update
 template <typename T>
-void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, int *active,
-                 bool xstate);
-void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *oactive,
-                        int first);
-void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int *active);
+void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate);
+// reduction finish modes (one workgroup per instance)
+enum { FIN_INIT64 = 0, FIN_V, FIN_S, FIN_T, FIN_X, FIN_IR_RES, FIN_INNER_INIT, FIN_FINALIZE };
+template <typename T>
+void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt);
+void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_copy_counter(hipStream_t s, const int *src, int *dst);
 void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
