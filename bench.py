@@ -4,18 +4,28 @@ Headline (default) workload: the mixed-precision batch -- 512 independent
 instances with uneven grids (each dimension uniform in 32..512, per-instance D
 seed and amplitude 0.1..10x) plus one 4096 x 4096 grid, advanced together by
 the batched path (FP32 multigrid-preconditioned BiCGSTAB inside FP64
-iterative refinement, FP64 assembly and residual). A step is one implicit time
-step of every instance. The plain FP64 path (the same kernels, one instance at
-a time, FP64 throughout) is timed on the same batch for the speed-up, and the
-mixed results are compared with it.
+iterative refinement; FP64 assembly, flux weights, residual and refinement).
+A step is one implicit time step of every instance; inputs are resident in HBM
+before the timed region starts.
 
-Prints one JSON line. --gpus above 1 is refused: this build targets one GPU.
+Beside it, on the same batch:
+  * the plain FP64 path -- the same kernels, one instance at a time, FP64
+    throughout -- timed for the speed-up;
+  * the error of the mixed result against the plain FP64 GPU result, and of
+    both against the CPU oracle on the smallest instances;
+  * the CPU oracle (banded-LU restatement of Solver::update, one thread) timed
+    on a bounded sample of the batch (cpu_baseline);
+  * HIP-event timing of the dominant kernel (the level-0 zebra line sweeps)
+    over the timed region, for the roofline.
+
+Prints one JSON line on stdout (progress on stderr). --gpus above 1 is refused.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -36,9 +46,8 @@ def log(*a):
 
 
 def make_batch_problems(n_inst, lo, hi, big, seed):
-    probs = []
-    for spec in sc.batch_specs(n_inst, lo, hi, seed):
-        probs.append(sc.seeded_case(spec["nx"], spec["ny"], seed=spec["seed"], scale=spec["scale"]))
+    probs = [sc.seeded_case(s["nx"], s["ny"], seed=s["seed"], scale=s["scale"])
+             for s in sc.batch_specs(n_inst, lo, hi, seed)]
     if big:
         probs.append(sc.seeded_case(big, big, seed=seed * 7 + 1, scale=1.0))
     return probs
@@ -49,25 +58,36 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
 
-def run_batched(probs, precision, warmup, steps, time_kernel=None):
-    b = sg.Batch(sg.make_opts(precision=precision))
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def run_mixed_batched(probs, warmup, steps):
+    b = sg.Batch(sg.make_opts(precision=sg.SG_MIXED))
     slots = [b.add(p) for p in probs]
     b.step(warmup)
-    if time_kernel:
-        b.time_kernel(time_kernel)
-    sync_t0 = time.perf_counter()
+    b.time_kernel("line_l0")
+    t0 = time.perf_counter()
     b.step(steps)
-    dt = time.perf_counter() - sync_t0
-    kt = b.kernel_time() if time_kernel else (0.0, 0)
-    iters = [b.stats(s)["last_iters"] for s in slots]
-    return b, slots, dt, kt, iters
+    dt = time.perf_counter() - t0
+    kms, klaunch, kbytes = b.kernel_time()
+    st = [b.stats(s) for s in slots]
+    b.close()
+    return dt, (kms, klaunch, kbytes), st
 
 
-def run_plain(probs, warmup, steps):
+def run_plain_fp64(probs, warmup, steps):
     """The plain FP64 path: the same kernels, one instance at a time."""
     total = 0.0
     fs = []
-    for p in probs:
+    for k, p in enumerate(probs):
         s = sg.Solver(p, sg.make_opts(precision=sg.SG_FP64))
         s.update(warmup)
         t0 = time.perf_counter()
@@ -75,7 +95,41 @@ def run_plain(probs, warmup, steps):
         total += time.perf_counter() - t0
         fs.append(s.f())
         s.close()
+        if k % 128 == 127:
+            log(f"[bench] plain FP64: {k + 1}/{len(probs)} instances")
     return total, fs
+
+
+def mixed_states(probs, nsteps):
+    b = sg.Batch(sg.make_opts(precision=sg.SG_MIXED))
+    slots = [b.add(p) for p in probs]
+    b.step(nsteps)
+    fs = [b.f(s) for s in slots]
+    b.close()
+    return fs
+
+
+def cpu_baseline(probs, budget_s):
+    """CPU oracle (banded LU, 1 thread) on instances of the batch in order,
+    one step each, until about budget_s seconds of CPU work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    cells = 0
+    t_total = 0.0
+    n = 0
+    for p in probs:
+        # banded LU costs N*nx^2: keep single steps under a few seconds
+        if p.ncells * p.nx * p.nx > 4e10:
+            continue
+        op = orc.OracleProblem(p)
+        t0 = time.process_time()
+        orc.step(op, p.f0)
+        t_total += time.process_time() - t0
+        cells += p.ncells
+        n += 1
+        if t_total >= budget_s:
+            break
+    return cells / t_total, n, cells, t_total
 
 
 def main():
@@ -88,14 +142,15 @@ def main():
     ap.add_argument("--hi", type=int, default=512)
     ap.add_argument("--big", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--oracle-checks", type=int, default=3)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run only the timed batched steps (for rocprofv3)")
+                    help="only the timed mixed batched steps (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
-              file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; the step of one GPU is what is measured "
+              "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if sg.lib().sg_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -104,23 +159,77 @@ def main():
     t0 = time.perf_counter()
     probs = make_batch_problems(args.instances, args.lo, args.hi, args.big, args.seed)
     cells = sum(p.ncells for p in probs)
-    log(f"[bench] {len(probs)} instances, {cells/1e6:.2f} M cells, setup {time.perf_counter()-t0:.1f}s")
+    log(f"[bench] {len(probs)} instances, {cells / 1e6:.2f} M cells, inputs in "
+        f"{time.perf_counter() - t0:.1f}s")
 
-    b, slots, dt, kt, iters = run_batched(probs, sg.SG_MIXED, args.warmup, args.steps)
+    dt, (kms, kl, kbytes), st = run_mixed_batched(probs, args.warmup, args.steps)
     value = cells * args.steps / dt
-    log(f"[bench] mixed batched: {dt/args.steps*1e3:.2f} ms/step, {value:.3e} cell-steps/s, "
-        f"mean inner iters {np.mean(iters):.1f}")
+    iters = [s["last_iters"] for s in st]
+    refs = [s["last_refinements"] for s in st]
+    log(f"[bench] mixed batched: {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} cell-steps/s, "
+        f"inner iterations mean {np.mean(iters):.1f} max {max(iters)}, refinements mean "
+        f"{np.mean(refs):.2f}")
     if args.profile_only:
-        b.close()
         return
+
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded smooth log-normal D)",
-           "config": {"workload": f"{args.instances} uneven instances ({args.lo}-{args.hi})^2"
-                                  f" + one {args.big}^2, mixed precision, batched",
-                      "instances": len(probs), "cells": cells}}
-    b.close()
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64",
+           "data": "synthetic: seeded smooth log-normal D tensor, G = sin cos p^2",
+           "config": {"workload": f"batched mixed precision: {args.instances} instances with "
+                                  f"dims uniform in {args.lo}..{args.hi} plus one {args.big}^2 "
+                                  f"grid, dt = 1e-3 day",
+                      "instances": len(probs), "cells": cells, "seed": args.seed},
+           "solver": {"inner_iterations_mean": float(np.mean(iters)), "inner_iterations_max": int(max(iters)),
+                      "refinements_mean": float(np.mean(refs)),
+                      "min_f": float(min(s["min_f"] for s in st)),
+                      "clamped_cells": int(sum(s["clamp_count"] for s in st)),
+                      "max_rel_residual": float(max(s["last_rel_residual"] for s in st))}}
+    if kl and kms > 0:
+        avg_ms = kms / kl
+        achieved = kbytes / (kms * 1e-3) / 1e9
+        out["roofline"] = {"kernel": "k_line<float,5,*> level-0 zebra line sweep (all instances)",
+                           "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "avg_launch_ms": avg_ms, "launches": kl,
+                           "algorithmic_bytes_per_launch": kbytes / kl}
+
+    if not args.skip_plain:
+        pw, ps = 1, 1
+        t_plain, f_plain = run_plain_fp64(probs, pw, ps)
+        plain_value = cells * ps / t_plain
+        log(f"[bench] plain FP64: {t_plain / ps * 1e3:.1f} ms/step, {plain_value:.3e} cell-steps/s")
+        f_mixed = mixed_states(probs, pw + ps)
+        errs = [rel_l2(a, b) for a, b in zip(f_mixed, f_plain)]
+        out["plain_fp64"] = {"value": plain_value, "unit": "cell-steps/s",
+                             "ms_per_step": t_plain / ps * 1e3,
+                             "path": "same kernels, one instance at a time, FP64"}
+        out["speedup_vs_plain_fp64"] = value / plain_value
+        acc = {"steps_compared": pw + ps, "max_rel_l2_mixed_vs_fp64": float(max(errs)),
+               "min_f_mixed": float(min(f.min() for f in f_mixed))}
+        if args.oracle_checks:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as orc
+            small = sorted(range(len(probs)), key=lambda k: probs[k].ncells)[:args.oracle_checks]
+            eo = []
+            for k in small:
+                ref = orc.run(orc.OracleProblem(probs[k]), probs[k].f0, pw + ps)
+                eo.append((rel_l2(f_mixed[k], ref), rel_l2(f_plain[k], ref)))
+            acc["oracle_instances"] = [list(probs[k].f0.shape) for k in small]
+            acc["max_rel_l2_mixed_vs_oracle"] = float(max(e[0] for e in eo))
+            acc["max_rel_l2_fp64_vs_oracle"] = float(max(e[1] for e in eo))
+        out["accuracy"] = acc
+        log(f"[bench] accuracy: {acc}")
+
+    if args.cpu_budget > 0:
+        v, n, c, t = cpu_baseline(probs, args.cpu_budget)
+        out["cpu_baseline"] = {"value": v, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"first {n} instances of the batch (banded-LU oracle "
+                                         f"small enough for single steps), 1 step each, "
+                                         f"{c} cells, {t:.1f} s CPU",
+                               "cpu": cpu_model(), "host_cpus": os.cpu_count()}
+        log(f"[bench] cpu baseline: {v:.3e} cell-steps/s")
     print(json.dumps(out), flush=True)
 
 

@@ -132,6 +132,7 @@ struct DevMaps {   // offsets into one device buffer
   int nsmap[kMaxLev][2][2], ncx[kMaxLev], ncy[kMaxLev];
   const LineGrp *gmap[kMaxLev][2][2], *gcx[kMaxLev], *gcy[kMaxLev];
   int ngmap[kMaxLev][2][2], ngcx[kMaxLev], ngcy[kMaxLev];
+  double gcells[kMaxLev][2][2];   // cells swept by one launch of a smoothing map
   int maxlev;
 };
 
@@ -167,6 +168,7 @@ struct Batch {
   size_t ev_used = 0;
   double t_ms = 0;
   int64_t t_launches = 0;
+  double t_bytes = 0;   // algorithmic bytes of the timed launches
 
   ~Batch();
   int init(const sg_opts *o, size_t budget);
@@ -514,6 +516,15 @@ int Batch::rebuild() {
     dmn.ngcx[l] = (int)M.gcx[l].size();
     dmn.gcy[l] = (const LineGrp *)put(M.gcy[l].data(), M.gcy[l].size(), sizeof(LineGrp));
     dmn.ngcy[l] = (int)M.gcy[l].size();
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) {
+        double c = 0;
+        for (const LineGrp &g : M.gmap[l][d][p]) {
+          const InstDev &I = table[g.inst];
+          c += (double)g.count * (d == 0 ? I.lev[l].nx : I.lev[l].ny);
+        }
+        dmn.gcells[l][d][p] = c;
+      }
   }
   dmn.maxlev = maxlev;
   if (bytes) HIPCHK(hipMemcpy(d_maps, host.data(), bytes, hipMemcpyHostToDevice));
@@ -570,6 +581,10 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
+      // algorithmic traffic of a sweep: per swept cell b, the lower band, the
+      // two factors, the two off-line couplings and x of the two neighbouring
+      // lines are read once and x is written once (9 values)
+      if (lid == time_id && l == 0) t_bytes += 9.0 * sizeof(T) * m.gcells[l][dir][p];
       tbegin(lid);
       launch_line<T>(stream, d_insts, m.gmap[l][dir][p], m.ngmap[l][dir][p], l, dir, zero, xid, bid);
       tend(lid);
@@ -881,13 +896,14 @@ int sg_batch_time_kernel(sg_batch *b, int kernel_id) {
   b->b.time_id = kernel_id;
   b->b.t_ms = 0;
   b->b.t_launches = 0;
+  b->b.t_bytes = 0;
   return SG_OK;
 }
 int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
   if (total_ms) *total_ms = b->b.t_ms;
   if (launches) *launches = b->b.t_launches;
-  if (bytes) *bytes = 0;
+  if (bytes) *bytes = b->b.t_bytes;
   return SG_OK;
 }
 const char *sg_kernel_name(int id) {

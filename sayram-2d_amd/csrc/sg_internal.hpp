@@ -33,6 +33,8 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   double scale;          // mixed: norm of the FP64 residual being corrected
   double onorm2, obnorm2, otol2;   // mixed: outer (FP64) residual norms
   double oprev2;         // mixed: residual norm^2 of the previous refinement
+  double best2;          // smallest Krylov residual norm^2 of this solve
+  int best_it, pad2;     // iteration at which it was reached
   double true_rel;       // finalize: ||R - M f|| / ||R||
   double minf;           // finalize: min f before clamping
   int iters, status;     // Krylov iteration count and status of this solve

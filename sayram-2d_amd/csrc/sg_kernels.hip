@@ -990,6 +990,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
     if (threadIdx.x == 0) {
       s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
       s.rnorm2 = t[0]; s.bnorm2 = t[1]; s.iters = 0;
+      s.best2 = t[0]; s.best_it = 0;
       if (t[0] <= s.tol2 * t[1]) { s.status = ST_DONE; atomicSub(cnt, 1); }
     }
   } else if (MODE == F_V) {
@@ -1021,7 +1022,13 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
         s.rho = (double)t[0];
         s.rnorm2 = (double)t[1];
         s.beta = (double)(((T)s.rho / (T)s.rho_old) * ((T)s.alpha / (T)s.omega));
-        if (s.rnorm2 <= s.tol2 * s.bnorm2) {
+        // Converged; or stalled at the rounding floor: no halving of the
+        // residual for 4 iterations while below 1e-9 relative (or 100x the
+        // tolerance, for the loose inner solves of mixed precision).
+        bool floor = false;
+        if (s.rnorm2 < 0.25 * s.best2) { s.best2 = s.rnorm2; s.best_it = s.iters; }
+        else floor = s.iters - s.best_it >= 4 && s.rnorm2 <= fmax(1e-18, 1e4 * s.tol2) * s.bnorm2;
+        if (s.rnorm2 <= s.tol2 * s.bnorm2 || floor) {
           s.status = ST_DONE;
           atomicSub(cnt, 1);
         } else if (s.iters >= s.maxit || !(s.rnorm2 == s.rnorm2) || s.rho == 0.0 ||
@@ -1039,10 +1046,12 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
       if (first) s.obnorm2 = t[1];
       s.onorm2 = t[0];
       s.scale = sqrt(t[0]);
-      // Converged; or stalled at the FP64 rounding floor of R - M x (the
-      // residual stopped halving within 100x of the tolerance).
+      // Converged; or stalled at the FP64 rounding floor of R - M x, which is
+      // about cond(M) eps and what a direct solve reaches too (the residual
+      // stopped halving while below 1e-9 relative).
       const bool conv = t[0] <= s.otol2 * s.obnorm2;
-      const bool floor = !first && t[0] > 0.25 * s.oprev2 && t[0] <= 1e4 * s.otol2 * s.obnorm2;
+      const bool floor = !first && t[0] > 0.25 * s.oprev2 &&
+                         t[0] <= fmax(1e-18, 1e4 * s.otol2) * s.obnorm2;
       s.oprev2 = t[0];
       if (conv || floor) {
         s.ostatus = ST_DONE;
@@ -1063,6 +1072,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
     if (threadIdx.x == 0) {
       s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
       s.rnorm2 = t[0]; s.bnorm2 = t[0]; s.iters = 0;
+      s.best2 = t[0]; s.best_it = 0;
       s.status = ST_ACTIVE;
     }
   } else {   // F_FINALIZE

@@ -285,9 +285,10 @@ class Batch:
         _check(lib().sg_batch_time_kernel(self.h, KERNEL_IDS[name] if name else -1))
 
     def kernel_time(self):
+        """(total ms, launches, algorithmic bytes) of the kernel chosen by time_kernel."""
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()
         _check(lib().sg_batch_kernel_time(self.h, C.byref(ms), C.byref(n), C.byref(b)))
-        return ms.value, n.value
+        return ms.value, n.value, b.value
 
     def close(self):
         if self.h:

@@ -1,0 +1,154 @@
+"""GPU behaviour beyond per-step parity: golden vectors, long trajectories,
+full-size properties (where the CPU oracle cannot follow), the error paths of
+the C ABI, resume, determinism and the batch lifecycle."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import sayram_cases as sc
+from conftest import rel_l2
+from test_oracle import load_golden
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(g)[:-4] for g in GOLDEN])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gpu_matches_golden(gpu_lib, path, precision):
+    p, z = load_golden(path)
+    s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=precision))
+    bands, R = s.debug_assemble()
+    scale = np.max(np.abs(z["bands0"]), axis=0)
+    assert np.all(np.abs(bands - z["bands0"]) <= 1e-11 * scale[None, :])
+    for k in range(z["f_steps"].shape[0]):
+        s.update(1)
+        assert rel_l2(s.f(), z["f_steps"][k]) <= 1e-12, f"step {k + 1}"
+    s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_long_trajectory_c1_grid(gpu_lib, precision):
+    """The reference's CPU-sized case: 90 (alpha0 5..90 deg) x 100 (ln E,
+    0.2..5 MeV), full tensor D, dt = 1e-3 day; 300 steps against the oracle,
+    checked every 10 steps at 1e-10 relative L2."""
+    p = sc.seeded_case(90, 100, seed=1, scale=1.0)
+    op = orc.OracleProblem(p)
+    s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=precision))
+    f = p.f0.copy()
+    worst = 0.0
+    for blk in range(30):
+        f = orc.run(op, f, 10)
+        s.update(10)
+        g = s.f()
+        worst = max(worst, rel_l2(g, f))
+        assert worst <= 1e-10, f"after {10 * (blk + 1)} steps: {worst:.3e}"
+        assert g.min() >= 0.0
+    s.close()
+
+
+def test_c2_size_properties(gpu_lib):
+    """2048 x 2048, tau = 1 day: true residual at the FP64 floor, f >= 0, and
+    the mixed path within 1e-10 of the FP64 path after 10 steps."""
+    p = sc.seeded_case(2048, 2048, seed=5, scale=1.0, inv_tau=1.0)
+    a = gpu_lib.Solver(p, gpu_lib.make_opts(precision=0))
+    b = gpu_lib.Solver(p, gpu_lib.make_opts(precision=1))
+    for _ in range(10):
+        a.update(1)
+        b.update(1)
+        assert a.stats()["last_rel_residual"] <= 1e-12
+        assert b.stats()["last_rel_residual"] <= 1e-12
+    fa, fb = a.f(), b.f()
+    assert fa.min() >= 0.0 and fb.min() >= 0.0
+    assert rel_l2(fb, fa) <= 1e-10
+    a.close()
+    b.close()
+
+
+def test_conservation_at_scale(gpu_lib):
+    """Zero-flux sides, no loss: sum G dx dy f is conserved at 1024^2."""
+    p = sc.seeded_case(1024, 1024, seed=6, scale=3.0)
+    p.bc_type = [sc.ZEROFLUX] * 4
+    w = p.G * np.outer(np.diff(p.xe), np.diff(p.ye))
+    m0 = np.sum(w * p.f0)
+    for prec in (0, 1):
+        s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=prec))
+        s.update(5)
+        assert abs(np.sum(w * s.f()) - m0) <= 1e-11 * m0
+        s.close()
+
+
+def test_resume_and_determinism(gpu_lib):
+    p = sc.seeded_case(70, 45, seed=7, scale=2.0)
+    a = gpu_lib.Solver(p)
+    a.update(6)
+    b = gpu_lib.Solver(p)
+    b.update(3)
+    f3 = b.f()
+    b.close()
+    c = gpu_lib.Solver(p)
+    c.set_f(f3, 3)          # resume = upload f and istep
+    c.update(3)
+    assert np.array_equal(c.f(), a.f())
+    assert abs(c.t() - 6 * p.dt) < 1e-15
+    d = gpu_lib.Solver(p)
+    d.update(6)
+    assert np.array_equal(d.f(), a.f())   # bit-reproducible
+    for s in (a, c, d):
+        s.close()
+
+
+def test_error_paths(gpu_lib):
+    sg = gpu_lib
+    p = sc.seeded_case(20, 10, seed=8)
+    q = sc.seeded_case(20, 10, seed=8)
+    q.xe = q.xe.copy()
+    q.xe[5] = q.xe[4]              # Grid2D::validate (source/Grid2D.h:44-67)
+    with pytest.raises(sg.SayramError) as e:
+        sg.Solver(q)
+    assert e.value.code == sg.SG_ERR_ARG and "strictly increasing" in str(e.value)
+    q = sc.seeded_case(20, 10, seed=8)
+    q.Dxx = q.Dxx.copy()
+    q.Dxx[3, 3] = np.nan
+    with pytest.raises(sg.SayramError) as e:
+        sg.Solver(q)
+    assert e.value.code == sg.SG_ERR_ARG
+    q = sc.seeded_case(20, 10, seed=8)
+    q.bc_val[2] = None              # Dirichlet side without values (source/Solver.cc:392-394)
+    with pytest.raises(sg.SayramError) as e:
+        sg.Solver(q)
+    assert e.value.code == sg.SG_ERR_BC
+    s = sg.Solver(p, sg.make_opts(max_iters=1, rtol=1e-16))
+    rc = s.step_status(1)
+    assert rc == sg.SG_ERR_NOCONV and s.stats()["status"] == sg.SG_ERR_NOCONV
+    assert s.f().min() >= 0.0
+    s.close()
+    b = sg.Batch(sg.make_opts(), hbm_budget=sg.Batch.instance_bytes(20, 10) + 1024)
+    assert b.add(p) == 0
+    assert b.try_add(p) is None          # over the HBM budget: nothing allocated
+    b.close()
+
+
+def test_batch_lifecycle_and_step_until(gpu_lib):
+    sg = gpu_lib
+    probs = [sc.seeded_case(30 + 7 * k, 20 + 5 * k, seed=40 + k, scale=1.0 + k) for k in range(4)]
+    b = sg.Batch()
+    slots = [b.add(p) for p in probs]
+    b.remove(slots[1])
+    assert b.size() == 3
+    s_new = b.add(probs[1])           # reuses the free slot
+    assert s_new == slots[1]
+    targets = np.array([2, 5, 3, 1], dtype=np.int64)
+    assert b.step_until(targets, 6) == 0
+    for sl, t, p in zip(slots, targets, probs):
+        assert b.stats(sl)["steps"] == t
+        ref = sg.Solver(p)
+        ref.update(int(t))
+        assert np.array_equal(b.f(sl), ref.f())
+        ref.close()
+    b.close()
