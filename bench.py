@@ -147,7 +147,19 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed mixed batched steps (for rocprofv3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times: the known-size kernel that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {a.numel() * 8} bytes 3 times")
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the step of one GPU is what is measured "
               "(run with --gpus 1)", file=sys.stderr)

@@ -153,6 +153,8 @@ struct Batch {
   int *d_part = nullptr;        // by compact index
   int *d_cnt = nullptr;         // [0] active, [1] oactive
   int *h_cnt = nullptr;         // pinned
+  int *h_pipe = nullptr;        // pinned, counters of the pipelined Krylov chunks
+  hipEvent_t ev_cnt[2] = {nullptr, nullptr};
   void *d_maps = nullptr;
   size_t d_maps_bytes = 0;
   DevMaps dm{};
@@ -202,6 +204,9 @@ Batch::~Batch() {
   (void)hipFree(d_insts); (void)hipFree(d_scal); (void)hipFree(d_ticket); (void)hipFree(d_part); (void)hipFree(d_cnt);
   (void)hipFree(d_maps); (void)hipFree(d_scratch);
   if (h_cnt) (void)hipHostFree(h_cnt);
+  if (h_pipe) (void)hipHostFree(h_pipe);
+  for (auto ev : ev_cnt)
+    if (ev) (void)hipEventDestroy(ev);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -227,6 +232,7 @@ int Batch::init(const sg_opts *o, size_t b) {
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&d_cnt, 2 * sizeof(int)));
   HIPCHK(hipHostMalloc((void **)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&h_pipe, 2 * sizeof(int), hipHostMallocDefault));
   return ensure_capacity(16);
 }
 
@@ -581,10 +587,11 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      // algorithmic traffic of a sweep: per swept cell b, the lower band, the
-      // two factors, the two off-line couplings and x of the two neighbouring
-      // lines are read once and x is written once (9 values)
-      if (lid == time_id && l == 0) t_bytes += 9.0 * sizeof(T) * m.gcells[l][dir][p];
+      // algorithmic traffic of a level-0 sweep: per swept cell b, the lower
+      // band, the two line factors and the two off-line couplings are read,
+      // one value of x of the other colour is read (each is shared by the two
+      // lines beside it) and x is written: 8 values
+      if (lid == time_id && l == 0) t_bytes += 8.0 * sizeof(T) * m.gcells[l][dir][p];
       tbegin(lid);
       launch_line<T>(stream, d_insts, m.gmap[l][dir][p], m.ngmap[l][dir][p], l, dir, zero, xid, bid);
       tend(lid);
@@ -630,14 +637,31 @@ template <typename T> void Batch::iteration(bool xstate) {
   launch_finish<T>(stream, FIN_X, d_insts, n, d_cnt);
 }
 
+// Krylov iterations in chunks of check_every. The next chunk is enqueued
+// before the host waits for the active-instance counter of the current one, so
+// the GPU never drains while the host decides; a chunk enqueued after the
+// last instance converged costs only early-exiting launches.
 template <typename T> int Batch::krylov(int maxit, bool xstate) {
-  int active = 0, rc;
-  if ((rc = read_counter(0, &active))) return rc;
-  for (int it = 0; it < maxit && active > 0;) {
-    const int chunk = std::min(opts.check_every, maxit - it);
-    for (int c = 0; c < chunk; ++c) iteration<T>(xstate);
-    it += chunk;
-    if ((rc = read_counter(0, &active))) return rc;
+  if (!ev_cnt[0]) {
+    for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&ev_cnt[k], hipEventDisableTiming));
+  }
+  auto chunk = [&](int slot, int &it) -> int {
+    const int c = std::min(opts.check_every, maxit - it);
+    for (int k = 0; k < c; ++k) iteration<T>(xstate);
+    it += c;
+    HIPCHK(hipMemcpyAsync(h_pipe + slot, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipEventRecord(ev_cnt[slot], stream));
+    return SG_OK;
+  };
+  int it = 0, slot = 0, rc;
+  if (maxit <= 0) return SG_OK;
+  if ((rc = chunk(slot, it))) return rc;
+  for (;;) {
+    const bool more = it < maxit;
+    if (more && (rc = chunk(slot ^ 1, it))) return rc;
+    HIPCHK(hipEventSynchronize(ev_cnt[slot]));
+    if (h_pipe[slot] == 0 || !more) break;
+    slot ^= 1;
   }
   return SG_OK;
 }

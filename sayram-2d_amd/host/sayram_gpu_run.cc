@@ -1,0 +1,250 @@
+// sayram_gpu_run -- the reference's driver (source/main.cc) on the gfx950 step.
+//
+//   sayram_gpu_run <p.ini> <problem.bin>
+//
+// p.ini is read with the reference's keys and derived values
+// (source/Parameters.cc:35-65): [basic] run_id, nalpha0, nE, alpha0min,
+// alpha0max (deg), Emin, Emax (MeV), T (days), nsteps; [diagnostics] nplots;
+// nsteps is rounded down to a multiple of nplots and dt = T / nsteps
+// (source/Parameters.h:44). The grid is uniform in (alpha0 [rad], ln E) as in
+// make_uniform (source/main.cc:20-37).
+//
+// problem.bin holds the Equation's per-cell inputs and boundary data for that
+// grid (written by sayram_cases.write_problem); an Equation subclass serves
+// them through the reference's accessors.
+//
+// Output, when built with HDF5: ./output/<run_id>/<run_id>_data.h5 with
+// /alpha0 (deg), /logEN (ln E - ln E0), /f/<k> (nx, ny) every nsteps/nplots
+// steps and /t = linspace(0, T, nplots + 1), as source/main.cc:57-89 writes.
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "GpuSolver.h"
+#include "sayram_host.h"
+
+#ifdef SAYRAM_HDF5
+#include <hdf5.h>
+#endif
+
+namespace {
+
+const double kPi = 3.141592653589793238462;   // source/common.h:39
+const double kE0 = 0.511875;                  // MeV, source/common.h:42
+
+// INI file with mINI's rules as the reference uses them: [sections], key =
+// value, ';' comments, lines without '=' ignored, keys case-insensitive.
+std::map<std::string, std::map<std::string, std::string>> read_ini(const std::string &path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("cannot open " + path);
+  std::map<std::string, std::map<std::string, std::string>> out;
+  std::string line, sec;
+  auto trim = [](std::string s) {
+    const char *ws = " \t\r\n";
+    s.erase(0, s.find_first_not_of(ws));
+    s.erase(s.find_last_not_of(ws) + 1);
+    return s;
+  };
+  auto lower = [](std::string s) {
+    for (char &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+  };
+  while (std::getline(in, line)) {
+    line = trim(line);
+    if (line.empty() || line[0] == ';') continue;
+    if (line[0] == '[') {
+      const auto e = line.find(']');
+      if (e != std::string::npos) sec = lower(trim(line.substr(1, e - 1)));
+      continue;
+    }
+    const auto eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    out[sec][lower(trim(line.substr(0, eq)))] = trim(line.substr(eq + 1));
+  }
+  return out;
+}
+
+struct Params {
+  std::string run_id;
+  std::size_t nalpha0 = 0, nE = 0;
+  double a0min = 0, a0max = 0, Emin = 0, Emax = 0, T = 0, nsteps = 0;
+  int nplots = 1, save_every = 1;
+  double dt() const { return T / nsteps; }
+};
+
+Params read_params(const std::string &path) {
+  auto ini = read_ini(path);
+  auto get = [&](const std::string &s, const std::string &k) {
+    auto it = ini.find(s);
+    if (it == ini.end()) throw std::runtime_error("section not found: " + s);
+    auto kv = it->second.find(k);
+    if (kv == it->second.end()) throw std::runtime_error("key not found: " + k);
+    return kv->second;
+  };
+  Params p;
+  p.run_id = get("basic", "run_id");
+  p.nalpha0 = std::stoul(get("basic", "nalpha0"));
+  p.nE = std::stoul(get("basic", "ne"));
+  p.a0min = std::stod(get("basic", "alpha0min"));
+  p.a0max = std::stod(get("basic", "alpha0max"));
+  p.Emin = std::stod(get("basic", "emin"));
+  p.Emax = std::stod(get("basic", "emax"));
+  p.T = std::stod(get("basic", "t"));
+  p.nsteps = std::stod(get("basic", "nsteps"));
+  p.nplots = std::stoi(get("diagnostics", "nplots"));
+  // source/Parameters.cc:59-60
+  p.save_every = (int)(p.nsteps / p.nplots);
+  p.nsteps = p.save_every * p.nplots;
+  return p;
+}
+
+Grid2D make_uniform(const Params &p) {   // source/main.cc:20-37
+  std::vector<double> xe(p.nalpha0 + 1), ye(p.nE + 1);
+  const double lo_x = p.a0min * kPi / 180, hi_x = p.a0max * kPi / 180;
+  const double lo_y = std::log(p.Emin), hi_y = std::log(p.Emax);
+  const double dx = (hi_x - lo_x) / (double)p.nalpha0, dy = (hi_y - lo_y) / (double)p.nE;
+  for (std::size_t i = 0; i <= p.nalpha0; ++i) xe[i] = lo_x + dx * (double)i;
+  for (std::size_t j = 0; j <= p.nE; ++j) ye[j] = lo_y + dy * (double)j;
+  return Grid2D(std::move(xe), std::move(ye));
+}
+
+// Equation served from arrays (problem.bin)
+class ArrayCase : public Equation {
+ public:
+  ArrayCase(const Mesh &m, const std::string &path) : Equation(m), nx_(m.nx()), ny_(m.ny()) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    char magic[8];
+    in.read(magic, 8);
+    if (std::memcmp(magic, "SGPROB1", 7) != 0) throw std::runtime_error("not a problem file");
+    std::int32_t n[2];
+    in.read(reinterpret_cast<char *>(n), sizeof(n));
+    if ((std::size_t)n[0] != nx_ || (std::size_t)n[1] != ny_)
+      throw std::runtime_error("problem file grid does not match the ini file");
+    auto rd = [&](std::vector<double> &v, std::size_t k) {
+      v.resize(k);
+      in.read(reinterpret_cast<char *>(v.data()), (std::streamsize)(k * sizeof(double)));
+    };
+    std::vector<double> tmp;
+    Array2D *dst[6] = {&G_, &Dxx_, &Dxy_, &Dyy_, &inv_tau_, &f0_};
+    f0_.resize({nx_, ny_});
+    for (Array2D *a : dst) {
+      rd(tmp, nx_ * ny_);
+      std::copy(tmp.begin(), tmp.end(), a->data());
+    }
+    std::int32_t bt[4];
+    in.read(reinterpret_cast<char *>(bt), sizeof(bt));
+    for (int s = 0; s < 4; ++s) {
+      type_[s] = bt[s] == 0 ? BCType::Dirichlet : BCType::ZeroFlux;
+      rd(val_[s], s < 2 ? ny_ + 1 : nx_ + 1);
+    }
+    if (!in) throw std::runtime_error("truncated problem file");
+  }
+  BCType bc_type(BoundaryID side) const override { return type_[(int)side]; }
+  double init_f(const Ind &ind) const override { return f0_(ind.i, ind.j); }
+  void update(double) override {}
+  bool dirichlet_vertex_value(BoundaryID side, std::size_t i, std::size_t j, double,
+                              double *out) const override {
+    const int s = (int)side;
+    if (type_[s] != BCType::Dirichlet) return false;
+    *out = val_[s][s < 2 ? j : i];
+    return true;
+  }
+
+ private:
+  std::size_t nx_, ny_;
+  Array2D f0_;
+  BCType type_[4];
+  std::vector<double> val_[4];
+};
+
+#ifdef SAYRAM_HDF5
+void h5_write(hid_t file, const std::string &name, const double *v, int rank, const hsize_t *dims) {
+  hid_t space = H5Screate_simple(rank, dims, nullptr);
+  hid_t lcpl = H5Pcreate(H5P_LINK_CREATE);
+  H5Pset_create_intermediate_group(lcpl, 1);
+  hid_t ds = H5Dcreate2(file, name.c_str(), H5T_NATIVE_DOUBLE, space, lcpl, H5P_DEFAULT, H5P_DEFAULT);
+  if (ds < 0) throw std::runtime_error("HDF5: cannot create " + name);
+  H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, v);
+  H5Dclose(ds);
+  H5Pclose(lcpl);
+  H5Sclose(space);
+}
+#endif
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc != 3) {
+    std::cerr << "usage: sayram_gpu_run <p.ini> <problem.bin>" << std::endl;
+    return 1;
+  }
+  try {
+    const Params paras = read_params(argv[1]);
+    Grid2D grid = make_uniform(paras);
+    Mesh m(grid, paras.dt());
+    ArrayCase eq(m, argv[2]);
+    GpuSolver solver(m, &eq);
+    const std::string out_dir = "./output/" + paras.run_id + "/";
+    ::mkdir("./output", 0755);
+    ::mkdir(out_dir.c_str(), 0755);
+    const std::size_t nx = m.nx(), ny = m.ny();
+#ifdef SAYRAM_HDF5
+    const std::string fname = out_dir + paras.run_id + "_data.h5";
+    hid_t file = H5Fcreate(fname.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) throw std::runtime_error("HDF5: cannot create " + fname);
+    std::vector<double> a0(nx), le(ny);
+    for (std::size_t i = 0; i < nx; ++i) a0[i] = m.x(i) / kPi * 180.0;
+    for (std::size_t j = 0; j < ny; ++j) le[j] = m.y(j) - std::log(kE0);
+    const hsize_t dnx[1] = {nx}, dny[1] = {ny}, dnf[2] = {nx, ny};
+    h5_write(file, "/alpha0", a0.data(), 1, dnx);
+    h5_write(file, "/logEN", le.data(), 1, dny);
+    h5_write(file, "/f/0", solver.f().data(), 2, dnf);
+#endif
+    const std::clock_t c0 = std::clock();
+    const auto w0 = std::chrono::steady_clock::now();
+    const int nsteps = (int)paras.nsteps;
+    for (int tstep = 1; tstep <= nsteps; ++tstep) {
+      solver.update();
+      if (tstep % paras.save_every == 0) {
+#ifdef SAYRAM_HDF5
+        h5_write(file, "/f/" + std::to_string(tstep / paras.save_every), solver.f().data(), 2, dnf);
+#endif
+      }
+    }
+    (void)solver.f();
+    const double wall =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+#ifdef SAYRAM_HDF5
+    std::vector<double> t(paras.nplots + 1);
+    for (int k = 0; k <= paras.nplots; ++k) t[k] = paras.T * k / paras.nplots;
+    const hsize_t dt_[1] = {t.size()};
+    h5_write(file, "/t", t.data(), 1, dt_);
+    H5Fclose(file);
+#endif
+    const sg_step_stats st = solver.stats();
+    std::cout << "CPU time used " << (double)(std::clock() - c0) / CLOCKS_PER_SEC << " seconds"
+              << std::endl;
+    std::cout << "wall time " << wall << " s, " << nsteps << " steps, "
+              << (double)nx * ny * nsteps / wall << " cell-steps/s, last iterations "
+              << st.last_iters << ", min f " << st.min_f << std::endl;
+    // the final state, for comparison by the caller
+    std::ofstream fo(out_dir + paras.run_id + "_final.bin", std::ios::binary);
+    fo.write(reinterpret_cast<const char *>(solver.f().data()),
+             (std::streamsize)(nx * ny * sizeof(double)));
+  } catch (const std::exception &e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 2;
+  }
+  return 0;
+}

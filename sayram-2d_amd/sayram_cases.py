@@ -127,3 +127,19 @@ def batch_specs(count: int, lo: int, hi: int, seed: int):
         s = float(10.0 ** rng.uniform(-1.0, 1.0))
         out.append(dict(nx=nx, ny=ny, seed=int(seed * 100003 + k), scale=s))
     return out
+
+
+def write_problem(p: Problem, path: str) -> None:
+    """Binary problem file read by the C++ driver (host/sayram_gpu_run.cc):
+    magic 'SGPROB1\\0', int32 nx, ny, then G, Dxx, Dxy, Dyy, inv_tau, f0 as
+    (nx, ny) float64 arrays, int32 bc types[4], and the four Dirichlet vertex
+    arrays (ny+1, ny+1, nx+1, nx+1 values; zeros for zero-flux sides)."""
+    with open(path, "wb") as fh:
+        fh.write(b"SGPROB1\0")
+        fh.write(np.array([p.nx, p.ny], dtype=np.int32).tobytes())
+        for a in (p.G, p.Dxx, p.Dxy, p.Dyy, p.inv_tau, p.f0):
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        fh.write(np.array(p.bc_type, dtype=np.int32).tobytes())
+        for s, n in zip(range(4), (p.ny + 1, p.ny + 1, p.nx + 1, p.nx + 1)):
+            v = p.bc_val[s] if p.bc_val[s] is not None else np.zeros(n)
+            fh.write(np.ascontiguousarray(v, dtype=np.float64).tobytes())
