@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -109,6 +110,29 @@ def mixed_states(probs, nsteps):
     return fs
 
 
+def serving(n_req, seed):
+    """Continuous batching under Poisson arrivals (mixed precision), on a
+    bounded seeded trace. The full serving workload (200 req/s, grids
+    log-uniform 32..2048 per side, 10..1000 steps) offers ~2.4e10 cell-steps/s,
+    far beyond one GPU, so its latency grows with the trace length; the trace
+    here keeps the arrival rate but draws grids in 32..512 and 10..100 steps,
+    and the sustainable rate of the full distribution is derived from the
+    measured serving throughput."""
+    import sayram_serve as sv
+    reqs = sv.poisson_trace(n_req, 200.0, 32, 512, 10, 100, seed=seed + 17)
+    loop = sv.ServeLoop(hbm_budget=64 << 30, precision=sg.SG_MIXED)
+    done, wall = loop.run(reqs, realtime=True)
+    loop.close()
+    s = sv.summarize(done, wall)
+    e_cells = ((2048 - 32) / math.log(2048 / 32)) ** 2      # log-uniform sides
+    e_steps = (10 + 1000) / 2
+    s.update({"trace": f"{n_req} requests, Poisson 200 req/s, sides log-uniform 32..512, "
+                       f"steps 10..100, mixed precision, HBM budget 64 GiB",
+              "sustainable_rate_full_distribution_req_per_s":
+                  s["throughput_cell_steps_per_s"] / (e_cells * e_steps)})
+    return s
+
+
 def cpu_baseline(probs, budget_s):
     """CPU oracle (banded LU, 1 thread) on instances of the batch in order,
     one step each, until about budget_s seconds of CPU work."""
@@ -145,6 +169,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--oracle-checks", type=int, default=3)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests in the serving trace (0 skips the serving measurement)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed mixed batched steps (for rocprofv3)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -233,6 +259,10 @@ def main():
             acc["max_rel_l2_fp64_vs_oracle"] = float(max(e[1] for e in eo))
         out["accuracy"] = acc
         log(f"[bench] accuracy: {acc}")
+
+    if args.serve_requests > 0:
+        out["serving"] = serving(args.serve_requests, args.seed)
+        log(f"[bench] serving: {out['serving']}")
 
     if args.cpu_budget > 0:
         v, n, c, t = cpu_baseline(probs, args.cpu_budget)

@@ -80,7 +80,7 @@ def test_driver_matches_oracle_and_writes_reference_layout(tmp_path):
     groups = re.findall(r'GROUP "(\d+)"|DATASET "(\d)"', hdr)
     snaps = sorted(int(b) for a, b in groups if b)
     assert snaps == [0, 1, 2, 3, 4, 5]
-    d = subprocess.run([H5DUMP, "-d", "/f/5", "-y", "-w", "0", str(h5)], capture_output=True,
+    d = subprocess.run([H5DUMP, "-d", "/f/5", "-m", "%.17g", "-y", "-w", "0", str(h5)], capture_output=True,
                        text=True).stdout
     body = d.split("DATA {", 1)[1].split("}", 1)[0]
     vals = np.array([float(x) for x in body.replace(",", " ").split()])
