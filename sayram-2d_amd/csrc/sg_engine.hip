@@ -122,16 +122,30 @@ struct Slot {
 struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
   std::vector<LineMap> smap[kMaxLev][2][2], cxmap[kMaxLev], cymap[kMaxLev];
-  std::vector<LineGrp> gmap[kMaxLev][2][2], gcx[kMaxLev], gcy[kMaxLev];
+  std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
+      gcy[kMaxLev][kNumTileShapes];
 };
+
+// Tile shape of a line solve by line length and direction: short lines in
+// 64-line x 16-step tiles, longer ones in 16 x 64, long rows in 4 x 256 (rows
+// stay coalesced along the line; columns keep >= 16 lines per tile so that
+// the accesses across the lines stay coalesced). It depends only on the
+// instance, so results do not depend on the batch.
+static int tile_shape(int n, int dir) {
+  if (n <= 24) return 0;
+  if (dir == 1 || n <= 160) return 1;
+  return 2;
+}
 
 struct DevMaps {   // offsets into one device buffer
   const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
   int ncmap[kMaxLev], npmap[kMaxLev];
   const LineMap *smap[kMaxLev][2][2], *cxmap[kMaxLev], *cymap[kMaxLev];
   int nsmap[kMaxLev][2][2], ncx[kMaxLev], ncy[kMaxLev];
-  const LineGrp *gmap[kMaxLev][2][2], *gcx[kMaxLev], *gcy[kMaxLev];
-  int ngmap[kMaxLev][2][2], ngcx[kMaxLev], ngcy[kMaxLev];
+  const LineGrp *gmap[kMaxLev][2][2][kNumTileShapes], *gcx[kMaxLev][kNumTileShapes],
+      *gcy[kMaxLev][kNumTileShapes];
+  int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
+      ngcy[kMaxLev][kNumTileShapes];
   double gcells[kMaxLev][2][2];   // cells swept by one launch of a smoothing map
   int maxlev;
 };
@@ -452,18 +466,25 @@ int Batch::rebuild() {
         if (!coarsest) M.pmap[l].push_back({c, k0});
       }
       if (coarsest) {
-        if (ly == 1) { M.cxmap[l].push_back({c, 0}); M.gcx[l].push_back({c, 0, 1, 1}); }
-        else { M.cymap[l].push_back({c, 0}); M.gcy[l].push_back({c, 0, 1, 1}); }
+        if (ly == 1) {
+          M.cxmap[l].push_back({c, 0});
+          M.gcx[l][tile_shape(lx, 0)].push_back({c, 0, 1, 1});
+        } else {
+          M.cymap[l].push_back({c, 0});
+          M.gcy[l][tile_shape(ly, 1)].push_back({c, 0, 1, 1});
+        }
       } else {
         for (int j = 0; j < ly; ++j) M.smap[l][0][j & 1].push_back({c, j});
         for (int i = 0; i < lx; ++i) M.smap[l][1][i & 1].push_back({c, i});
-        // groups of up to 64 lines of one colour
+        // groups of kTileElems / S lines of one colour
         for (int d = 0; d < 2; ++d) {
-          const int nl = d == 0 ? ly : lx;
+          const int nl = d == 0 ? ly : lx;          // number of lines
+          const int sh = tile_shape(d == 0 ? lx : ly, d);
+          const int per = kTileElems / tile_steps(sh);
           for (int p = 0; p < 2; ++p)
-            for (int f = p; f < nl; f += 2 * kLineBlock) {
-              const int cntl = std::min(kLineBlock, (nl - f + 1) / 2);
-              M.gmap[l][d][p].push_back({c, f, cntl, 2});
+            for (int f = p; f < nl; f += 2 * per) {
+              const int cntl = std::min(per, (nl - f + 1) / 2);
+              M.gmap[l][d][p][sh].push_back({c, f, cntl, 2});
             }
         }
       }
@@ -477,9 +498,12 @@ int Batch::rebuild() {
     for (int d = 0; d < 2; ++d)
       for (int p = 0; p < 2; ++p) sz(M.smap[l][d][p].size(), sizeof(LineMap));
     sz(M.cxmap[l].size(), sizeof(LineMap)); sz(M.cymap[l].size(), sizeof(LineMap));
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p) sz(M.gmap[l][d][p].size(), sizeof(LineGrp));
-    sz(M.gcx[l].size(), sizeof(LineGrp)); sz(M.gcy[l].size(), sizeof(LineGrp));
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      for (int d = 0; d < 2; ++d)
+        for (int p = 0; p < 2; ++p) sz(M.gmap[l][d][p][sh].size(), sizeof(LineGrp));
+      sz(M.gcx[l][sh].size(), sizeof(LineGrp));
+      sz(M.gcy[l][sh].size(), sizeof(LineGrp));
+    }
   }
   HIPCHK(hipStreamSynchronize(stream));
   if (bytes > d_maps_bytes) {
@@ -512,23 +536,26 @@ int Batch::rebuild() {
     dmn.ncx[l] = (int)M.cxmap[l].size();
     dmn.cymap[l] = (const LineMap *)put(M.cymap[l].data(), M.cymap[l].size(), sizeof(LineMap));
     dmn.ncy[l] = (int)M.cymap[l].size();
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p) {
-        dmn.gmap[l][d][p] = (const LineGrp *)put(M.gmap[l][d][p].data(), M.gmap[l][d][p].size(),
-                                                  sizeof(LineGrp));
-        dmn.ngmap[l][d][p] = (int)M.gmap[l][d][p].size();
-      }
-    dmn.gcx[l] = (const LineGrp *)put(M.gcx[l].data(), M.gcx[l].size(), sizeof(LineGrp));
-    dmn.ngcx[l] = (int)M.gcx[l].size();
-    dmn.gcy[l] = (const LineGrp *)put(M.gcy[l].data(), M.gcy[l].size(), sizeof(LineGrp));
-    dmn.ngcy[l] = (int)M.gcy[l].size();
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      for (int d = 0; d < 2; ++d)
+        for (int p = 0; p < 2; ++p) {
+          auto &v = M.gmap[l][d][p][sh];
+          dmn.gmap[l][d][p][sh] = (const LineGrp *)put(v.data(), v.size(), sizeof(LineGrp));
+          dmn.ngmap[l][d][p][sh] = (int)v.size();
+        }
+      dmn.gcx[l][sh] = (const LineGrp *)put(M.gcx[l][sh].data(), M.gcx[l][sh].size(), sizeof(LineGrp));
+      dmn.ngcx[l][sh] = (int)M.gcx[l][sh].size();
+      dmn.gcy[l][sh] = (const LineGrp *)put(M.gcy[l][sh].data(), M.gcy[l][sh].size(), sizeof(LineGrp));
+      dmn.ngcy[l][sh] = (int)M.gcy[l][sh].size();
+    }
     for (int d = 0; d < 2; ++d)
       for (int p = 0; p < 2; ++p) {
         double c = 0;
-        for (const LineGrp &g : M.gmap[l][d][p]) {
-          const InstDev &I = table[g.inst];
-          c += (double)g.count * (d == 0 ? I.lev[l].nx : I.lev[l].ny);
-        }
+        for (int sh = 0; sh < kNumTileShapes; ++sh)
+          for (const LineGrp &g : M.gmap[l][d][p][sh]) {
+            const InstDev &I = table[g.inst];
+            c += (double)g.count * (d == 0 ? I.lev[l].nx : I.lev[l].ny);
+          }
         dmn.gcells[l][d][p] = c;
       }
   }
@@ -593,7 +620,9 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
       // lines beside it) and x is written: 8 values
       if (lid == time_id && l == 0) t_bytes += 8.0 * sizeof(T) * m.gcells[l][dir][p];
       tbegin(lid);
-      launch_line<T>(stream, d_insts, m.gmap[l][dir][p], m.ngmap[l][dir][p], l, dir, zero, xid, bid);
+      for (int sh = 0; sh < kNumTileShapes; ++sh)
+        launch_line<T>(stream, d_insts, m.gmap[l][dir][p][sh], m.ngmap[l][dir][p][sh], l, dir, sh,
+                       zero, xid, bid);
       tend(lid);
       zero = false;
     }
@@ -603,8 +632,10 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const bool pre_zero = zero;
   // exact solve on the coarsest level of the instances that end here
   tbegin(3);
-  launch_line<T>(stream, d_insts, m.gcx[l], m.ngcx[l], l, 0, true, xid, bid);
-  launch_line<T>(stream, d_insts, m.gcy[l], m.ngcy[l], l, 1, true, xid, bid);
+  for (int sh = 0; sh < kNumTileShapes; ++sh) {
+    launch_line<T>(stream, d_insts, m.gcx[l][sh], m.ngcx[l][sh], l, 0, sh, true, xid, bid);
+    launch_line<T>(stream, d_insts, m.gcy[l][sh], m.ngcy[l][sh], l, 1, sh, true, xid, bid);
+  }
   tend(3);
   if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
     tbegin(4);

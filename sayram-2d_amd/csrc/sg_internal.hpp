@@ -14,8 +14,10 @@ namespace sg {
 constexpr int kMaxLev = 16;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
 constexpr int kLineBlock = 64;  // lines per workgroup in line kernels (one wave)
-constexpr int kTile = 16;       // steps along the line per LDS tile
-constexpr int kLineThreads = 256;   // threads of a line workgroup (4 per tile step of a line)
+constexpr int kLineThreads = 256;   // threads of a line-solve workgroup
+constexpr int kTileElems = 1024;    // elements of one LDS tile (lines x steps), 4 per thread
+constexpr int kNumTileShapes = 3;   // tile steps S = 16, 64, 256 (lines 64, 16, 4)
+__host__ __device__ constexpr int tile_steps(int sc) { return sc == 0 ? 16 : sc == 1 ? 64 : 256; }
 
 // band slots; level 0 uses the first five (the operator is 5-point), coarse
 // Galerkin levels use all nine.

@@ -465,60 +465,70 @@ __global__ void __launch_bounds__(kLineBlock) k_line_factor(const InstDev *__res
   }
 }
 
-// Zebra line Gauss-Seidel. A workgroup solves a group of up to 64 lines of one
-// colour exactly with the precomputed factors, the off-line couplings moved to
-// the right-hand side. The lines are walked in tiles of kTile steps staged
-// through LDS: the 256 threads load a 64 x kTile tile with coalesced global
-// accesses (along the line for rows, across the lines for columns), every
-// thread issuing the loads of its 4 elements together, then the first wave runs
-// the 64 recurrences on the tile from LDS (one lane per line). The forward pass
-// stores the eliminated right-hand side in x (no other line of this colour
-// reads it), the backward pass turns it into the solution. ZERO: the other
-// lines are zero (first sweep of a cycle).
-template <typename T, int NB, int DIR, bool ZERO>
+// Zebra line Gauss-Seidel. A workgroup solves a group of lines of one colour
+// exactly with the precomputed factors, the off-line couplings moved to the
+// right-hand side. The lines are walked in tiles of L lines x S steps
+// (L*S = 1024; S = 16, 64 or 256 by line length) staged through LDS: the 256
+// threads load a tile with coalesced global accesses (along the lines for
+// rows, across the lines for columns), then solve it with the threads laid out
+// along the lines: both Thomas recurrences are first-order affine maps
+// (d_t = a_t d_{t-1} + b_t forward, x_t = d_t - u_t x_{t+1} backward), so
+// every thread composes the maps of its 4 consecutive steps, a segmented wave
+// scan combines them across the S/4 threads of a line, and the carry from the
+// previous tile enters through the composed map. The forward pass parks the
+// eliminated right-hand side in x (no other line of this colour reads it);
+// the backward pass turns it into the solution. ZERO: the other lines are
+// zero (first sweep of a cycle).
+template <typename T, int NB, int DIR, bool ZERO, int S>
 __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict__ insts,
                                                       const LineGrp *__restrict__ map, int l,
                                                       int xid, int bid) {
-  constexpr int Q = kLineBlock * kTile / kLineThreads;   // tile elements per thread
-  __shared__ T s_r[kLineBlock][kTile + 1];
-  __shared__ T s_a[kLineBlock][kTile + 1];
-  __shared__ T s_i[kLineBlock][kTile + 1];
+  constexpr int L = kTileElems / S;          // lines per group
+  constexpr int W = S / 4;                   // threads per line in the solve
+  constexpr int Q = kTileElems / kLineThreads;
+  __shared__ T s_r[L][S + 1];
+  __shared__ T s_a[L][S + 1];
+  __shared__ T s_i[L][S + 1];
+  __shared__ T s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
   if (I.sc->status != ST_ACTIVE) return;
-  const Lev &L = I.lev[l];
-  const int nx = L.nx, ny = L.ny;
+  const Lev &Lv = I.lev[l];
+  const int nx = Lv.nx, ny = Lv.ny;
   T *x; const T *b;
   level_xb<T>(I, l, xid, bid, x, b);
-  const T *__restrict__ lo = cptr<T>(L.band[DIR == 0 ? BW : BS]);
-  const T *__restrict__ om = cptr<T>(L.band[DIR == 0 ? BS : BW]);
-  const T *__restrict__ op = cptr<T>(L.band[DIR == 0 ? BN : BE]);
+  const T *__restrict__ lo = cptr<T>(Lv.band[DIR == 0 ? BW : BS]);
+  const T *__restrict__ om = cptr<T>(Lv.band[DIR == 0 ? BS : BW]);
+  const T *__restrict__ op = cptr<T>(Lv.band[DIR == 0 ? BN : BE]);
   const T *omm = nullptr, *omp = nullptr, *opm = nullptr, *opp = nullptr;
   if (NB == 9) {   // diagonal couplings: (minus side, along -1/+1), (plus side, along -1/+1)
-    omm = cptr<T>(L.band[BSW]);
-    omp = cptr<T>(L.band[DIR == 0 ? BSE : BNW]);
-    opm = cptr<T>(L.band[DIR == 0 ? BNW : BSE]);
-    opp = cptr<T>(L.band[BNE]);
+    omm = cptr<T>(Lv.band[BSW]);
+    omp = cptr<T>(Lv.band[DIR == 0 ? BSE : BNW]);
+    opm = cptr<T>(Lv.band[DIR == 0 ? BNW : BSE]);
+    opp = cptr<T>(Lv.band[BNE]);
   }
-  const T *__restrict__ inv = cptr<T>(L.linv[DIR]);
-  const T *__restrict__ cu = cptr<T>(L.lup[DIR]);
+  const T *__restrict__ inv = cptr<T>(Lv.linv[DIR]);
+  const T *__restrict__ cu = cptr<T>(Lv.lup[DIR]);
   const int n = DIR == 0 ? nx : ny;            // line length
   const int nperp = DIR == 0 ? ny : nx;
   const size_t st = DIR == 0 ? 1 : (size_t)nx;  // along the line
   const size_t so = DIR == 0 ? (size_t)nx : 1;  // to the neighbouring lines
   const int cnt = gr.count;
   const int tid = threadIdx.x;
-  // element e of a tile -> (line k, step u), in coalesced order in memory
+  // load/store layout: element e -> (line k, step u), coalesced in memory
   int ek[Q], eu[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int e = q * kLineThreads + tid;
-    ek[q] = DIR == 0 ? e / kTile : e % kLineBlock;
-    eu[q] = DIR == 0 ? e % kTile : e / kLineBlock;
+    ek[q] = DIR == 0 ? e / S : e % L;
+    eu[q] = DIR == 0 ? e % S : e / L;
   }
-  T dprev = T(0);
-  for (int t0 = 0; t0 < n; t0 += kTile) {
-    const int tw = min(kTile, n - t0);
+  // solve layout: thread -> (line sk, steps 4*sc .. 4*sc+3)
+  const int sk = tid / W, sc = tid % W;
+  const bool sk_ok = sk < cnt;
+  if (tid < L) s_c[tid] = T(0);
+  for (int t0 = 0; t0 < n; t0 += S) {
+    const int tw = min(S, n - t0);
     T vb[Q], vlo[Q], viv[Q], vom[Q], vop[Q], xm[Q], xp[Q];
     T dmm[Q], dmp[Q], dpm[Q], dpp[Q], xmm[Q], xmp[Q], xpm[Q], xpp[Q];
 #pragma unroll
@@ -559,11 +569,36 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
       s_i[ek[q]][eu[q]] = viv[q];
     }
     __syncthreads();
-    if (tid < cnt) {
-      for (int u = 0; u < tw; ++u) {
-        dprev = (s_r[tid][u] - s_a[tid][u] * dprev) * s_i[tid][u];
-        s_r[tid][u] = dprev;
+    {   // forward: d_t = a_t d_{t-1} + b_t with a_t = -lo_t inv_t, b_t = r_t inv_t
+      T al[4], be[4];
+      T A = T(1), B = T(0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int u = 4 * sc + e;
+        al[e] = T(1); be[e] = T(0);
+        if (sk_ok && u < tw) {
+          al[e] = -s_a[sk][u] * s_i[sk][u];
+          be[e] = s_r[sk][u] * s_i[sk][u];
+        }
+        B = al[e] * B + be[e];
+        A = al[e] * A;
       }
+#pragma unroll
+      for (int off = 1; off < W; off <<= 1) {
+        const T Ap = __shfl_up(A, off, W), Bp = __shfl_up(B, off, W);
+        if (sc >= off) { B = A * Bp + B; A = A * Ap; }
+      }
+      T Ae = __shfl_up(A, 1, W), Be = __shfl_up(B, 1, W);
+      if (sc == 0) { Ae = T(1); Be = T(0); }
+      T d = Ae * (sk_ok ? s_c[sk] : T(0)) + Be;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int u = 4 * sc + e;
+        d = al[e] * d + be[e];
+        if (sk_ok && u < tw) s_r[sk][u] = d;
+      }
+      __syncthreads();
+      if (sk_ok && sc == W - 1) s_c[sk] = d;
     }
     __syncthreads();
 #pragma unroll
@@ -574,11 +609,11 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
     }
     __syncthreads();
   }
-  T xn = T(0);
-  const int ntiles = (n + kTile - 1) / kTile;
+  if (tid < L) s_c[tid] = T(0);
+  const int ntiles = (n + S - 1) / S;
   for (int tt = ntiles - 1; tt >= 0; --tt) {
-    const int t0 = tt * kTile;
-    const int tw = min(kTile, n - t0);
+    const int t0 = tt * S;
+    const int tw = min(S, n - t0);
     T vd[Q], vc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -596,11 +631,36 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
       s_a[ek[q]][eu[q]] = vc[q];
     }
     __syncthreads();
-    if (tid < cnt) {
-      for (int u = tw - 1; u >= 0; --u) {
-        xn = s_r[tid][u] - s_a[tid][u] * xn;
-        s_r[tid][u] = xn;
+    {   // backward: x_t = -u_t x_{t+1} + d_t, composed from the line's end
+      T al[4], be[4];
+      T A = T(1), B = T(0);
+#pragma unroll
+      for (int e = 3; e >= 0; --e) {
+        const int u = 4 * sc + e;
+        al[e] = T(1); be[e] = T(0);
+        if (sk_ok && u < tw) {
+          al[e] = -s_a[sk][u];
+          be[e] = s_r[sk][u];
+        }
+        B = al[e] * B + be[e];
+        A = al[e] * A;
       }
+#pragma unroll
+      for (int off = 1; off < W; off <<= 1) {
+        const T An = __shfl_down(A, off, W), Bn = __shfl_down(B, off, W);
+        if (sc + off < W) { B = A * Bn + B; A = A * An; }
+      }
+      T Ae = __shfl_down(A, 1, W), Be = __shfl_down(B, 1, W);
+      if (sc == W - 1) { Ae = T(1); Be = T(0); }
+      T xv = Ae * (sk_ok ? s_c[sk] : T(0)) + Be;
+#pragma unroll
+      for (int e = 3; e >= 0; --e) {
+        const int u = 4 * sc + e;
+        xv = al[e] * xv + be[e];
+        if (sk_ok && u < tw) s_r[sk][u] = xv;
+      }
+      __syncthreads();
+      if (sk_ok && sc == 0) s_c[sk] = xv;
     }
     __syncthreads();
 #pragma unroll
@@ -1164,12 +1224,11 @@ void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n
   }
 }
 
-template <typename T>
-void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
-                 bool zero, int xid, int bid) {
-  if (!ngroups) return;
+template <typename T, int S>
+static void launch_line_s(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
+                          int dir, bool zero, int xid, int bid) {
   const bool nine = l > 0;
-#define SG_LINE(NB, DIR, Z) k_line<T, NB, DIR, Z><<<ngroups, kLineThreads, 0, s>>>(d, m, l, xid, bid)
+#define SG_LINE(NB, DIR, Z) k_line<T, NB, DIR, Z, S><<<ngroups, kLineThreads, 0, s>>>(d, m, l, xid, bid)
   if (nine) {
     if (dir == 0) { if (zero) SG_LINE(9, 0, true); else SG_LINE(9, 0, false); }
     else { if (zero) SG_LINE(9, 1, true); else SG_LINE(9, 1, false); }
@@ -1179,10 +1238,19 @@ void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups,
   }
 #undef SG_LINE
 }
+
+template <typename T>
+void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
+                 int shape, bool zero, int xid, int bid) {
+  if (!ngroups) return;
+  if (shape == 0) launch_line_s<T, 16>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else if (shape == 1) launch_line_s<T, 64>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else launch_line_s<T, 256>(s, d, m, ngroups, l, dir, zero, xid, bid);
+}
 template void launch_line<double>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
-                                  bool, int, int);
+                                  int, bool, int, int);
 template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
-                                 bool, int, int);
+                                 int, bool, int, int);
 
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,

@@ -12,9 +12,10 @@ void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, i
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
                         bool mixed);
+// shape: tile steps 16, 64 or 256 (lines per group 64, 16, 4)
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
-                 bool zero, int xid, int bid);
+                 int shape, bool zero, int xid, int bid);
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                      int xid, int bid);
