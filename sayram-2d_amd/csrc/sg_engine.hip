@@ -137,6 +137,7 @@ static int tile_shape(int n, int dir) {
   return 2;
 }
 
+
 struct DevMaps {   // offsets into one device buffer
   const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
   int ncmap[kMaxLev], npmap[kMaxLev];
@@ -724,11 +725,14 @@ int Batch::step_once() {
     tbegin(1);
     // LU factors of the line matrices: smoothing lines of both colours and the
     // exact solve on the coarsest level
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p)
-        launch_line_factor(stream, d_insts, dm.smap[l][d][p], dm.nsmap[l][d][p], l, d, mixed);
-    launch_line_factor(stream, d_insts, dm.cxmap[l], dm.ncx[l], l, 0, mixed);
-    launch_line_factor(stream, d_insts, dm.cymap[l], dm.ncy[l], l, 1, mixed);
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      for (int d = 0; d < 2; ++d)
+        for (int p = 0; p < 2; ++p)
+          launch_line_factor(stream, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d,
+                             sh, mixed);
+      launch_line_factor(stream, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, mixed);
+      launch_line_factor(stream, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, mixed);
+    }
     if (l + 1 < dm.maxlev) {
       launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, mixed);
       launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed);

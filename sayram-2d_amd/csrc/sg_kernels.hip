@@ -435,33 +435,132 @@ __device__ __forceinline__ void level_xb(const InstDev &I, int l, int xid, int b
 // once per step: lup_t = upper_t / pivot_t, linv_t = 1 / pivot_t with
 // pivot_t = diag_t - lower_t lup_{t-1} (Thomas elimination without pivoting;
 // the level-0 line matrices are column diagonally dominant M-matrices).
-template <typename T, int DIR>
-__global__ void __launch_bounds__(kLineBlock) k_line_factor(const InstDev *__restrict__ insts,
-                                                           const LineMap *__restrict__ map,
-                                                           int nlines, int l) {
-  const int g = blockIdx.x * kLineBlock + threadIdx.x;
-  if (g >= nlines) return;
-  const LineMap m = map[g];
-  const InstDev &I = insts[m.inst];
+// Same tiles and layout as the sweeps below. The recurrence
+// lup_t = c_t / (b_t - a_t lup_{t-1}) is a Moebius map, i.e. the 2x2 matrix
+// [[0, c_t], [-a_t, b_t]] acting on (P, Q) with lup = P / Q, so the threads of
+// a line compose their 4 steps' matrices (rescaled to stay in range), scan
+// them across the line and carry (P, Q) between tiles; each thread then runs
+// the exact recurrence over its 4 steps from the value the scan gives it.
+template <typename T, int DIR, int S>
+__global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__restrict__ insts,
+                                                             const LineGrp *__restrict__ map,
+                                                             int l) {
+  constexpr int L = kTileElems / S;
+  constexpr int W = S / 4;
+  constexpr int Q = kTileElems / kLineThreads;
+  __shared__ double s_lo[L][S + 1];
+  __shared__ double s_d[L][S + 1];
+  __shared__ double s_up[L][S + 1];
+  __shared__ double s_c[L];
+  const LineGrp gr = map[blockIdx.x];
+  const InstDev &I = insts[gr.inst];
   if (I.sc->ostatus == ST_IDLE) return;
-  const Lev &L = I.lev[l];
-  const int nx = L.nx, ny = L.ny;
+  const Lev &Lv = I.lev[l];
+  const int nx = Lv.nx, ny = Lv.ny;
+  const double *__restrict__ bc = Lv.band64[BC];
+  const double *__restrict__ lo = Lv.band64[DIR == 0 ? BW : BS];
+  const double *__restrict__ up = Lv.band64[DIR == 0 ? BE : BN];
+  T *__restrict__ inv = ptr<T>(Lv.linv[DIR]);
+  T *__restrict__ cu = ptr<T>(Lv.lup[DIR]);
   const int n = DIR == 0 ? nx : ny;
-  const size_t k0 = DIR == 0 ? (size_t)m.line * nx : (size_t)m.line;
   const size_t st = DIR == 0 ? 1 : (size_t)nx;
-  const double *__restrict__ bc = L.band64[BC];
-  const double *__restrict__ lo = L.band64[DIR == 0 ? BW : BS];
-  const double *__restrict__ up = L.band64[DIR == 0 ? BE : BN];
-  T *__restrict__ inv = ptr<T>(L.linv[DIR]);
-  T *__restrict__ cu = ptr<T>(L.lup[DIR]);
-  double cprev = 0.0;
-  for (int t = 0; t < n; ++t) {
-    const size_t K = k0 + (size_t)t * st;
-    const double a = t > 0 ? lo[K] : 0.0;
-    const double iv = 1.0 / (bc[K] - a * cprev);
-    cprev = t < n - 1 ? up[K] * iv : 0.0;
-    inv[K] = (T)iv;
-    cu[K] = (T)cprev;
+  const size_t so = DIR == 0 ? (size_t)nx : 1;
+  const int cnt = gr.count;
+  const int tid = threadIdx.x;
+  int ek[Q], eu[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = q * kLineThreads + tid;
+    ek[q] = DIR == 0 ? e / S : e % L;
+    eu[q] = DIR == 0 ? e % S : e / L;
+  }
+  const int sk = tid / W, sc = tid % W;
+  const bool sk_ok = sk < cnt;
+  if (tid < L) s_c[tid] = 0.0;
+  for (int t0 = 0; t0 < n; t0 += S) {
+    const int tw = min(S, n - t0);
+    double vl[Q], vd[Q], vu[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = ek[q], u = eu[q];
+      vl[q] = 0.0; vd[q] = 1.0; vu[q] = 0.0;
+      if (k < cnt && u < tw) {
+        const int t = t0 + u;
+        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)t * st;
+        vl[q] = t > 0 ? lo[K] : 0.0;
+        vd[q] = bc[K];
+        vu[q] = t < n - 1 ? up[K] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      s_lo[ek[q]][eu[q]] = vl[q];
+      s_d[ek[q]][eu[q]] = vd[q];
+      s_up[ek[q]][eu[q]] = vu[q];
+    }
+    __syncthreads();
+    // compose this thread's 4 maps: (P, Q) <- [[0, c], [-a, b]] (P, Q)
+    double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = 4 * sc + e;
+      if (sk_ok && u < tw) {
+        const double a = s_lo[sk][u], bb = s_d[sk][u], c = s_up[sk][u];
+        const double n00 = c * m10, n01 = c * m11;
+        const double n10 = -a * m00 + bb * m10, n11 = -a * m01 + bb * m11;
+        const double sc_ = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+        m00 = n00 * sc_; m01 = n01 * sc_; m10 = n10 * sc_; m11 = n11 * sc_;
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < W; off <<= 1) {
+      const double p00 = __shfl_up(m00, off, W), p01 = __shfl_up(m01, off, W);
+      const double p10 = __shfl_up(m10, off, W), p11 = __shfl_up(m11, off, W);
+      if (sc >= off) {   // own * previous
+        const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+        const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+        const double sc_ = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+        m00 = n00 * sc_; m01 = n01 * sc_; m10 = n10 * sc_; m11 = n11 * sc_;
+      }
+    }
+    double e00 = __shfl_up(m00, 1, W), e01 = __shfl_up(m01, 1, W);
+    double e10 = __shfl_up(m10, 1, W), e11 = __shfl_up(m11, 1, W);
+    if (sc == 0) { e00 = 1.0; e01 = 0.0; e10 = 0.0; e11 = 1.0; }
+    // lup before this thread's first step: the exclusive map applied to the
+    // carry (P, Q) = (lup_carry, 1)
+    const double c0 = sk_ok ? s_c[sk] : 0.0;
+    const double P = e00 * c0 + e01, Qv = e10 * c0 + e11;
+    double cprev = Qv != 0.0 ? P / Qv : 0.0;
+    double ivs[4], cus[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = 4 * sc + e;
+      ivs[e] = 0.0; cus[e] = 0.0;
+      if (sk_ok && u < tw) {
+        const double iv = 1.0 / (s_d[sk][u] - s_lo[sk][u] * cprev);
+        cprev = s_up[sk][u] * iv;
+        ivs[e] = iv;
+        cus[e] = cprev;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int u = 4 * sc + e;
+      if (sk_ok && u < tw) { s_d[sk][u] = ivs[e]; s_up[sk][u] = cus[e]; }
+    }
+    if (sk_ok && sc == W - 1) s_c[sk] = cprev;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = ek[q], u = eu[q];
+      if (k < cnt && u < tw) {
+        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st;
+        inv[K] = (T)s_d[k][u];
+        cu[K] = (T)s_up[k][u];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1211,17 +1310,16 @@ void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l,
   }
 }
 
-void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
-                        bool mixed) {
-  if (!n) return;
-  const unsigned g = nblocks(n, kLineBlock);
-  if (mixed) {
-    if (dir == 0) k_line_factor<float, 0><<<g, kLineBlock, 0, s>>>(d, m, n, l);
-    else k_line_factor<float, 1><<<g, kLineBlock, 0, s>>>(d, m, n, l);
-  } else {
-    if (dir == 0) k_line_factor<double, 0><<<g, kLineBlock, 0, s>>>(d, m, n, l);
-    else k_line_factor<double, 1><<<g, kLineBlock, 0, s>>>(d, m, n, l);
-  }
+void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
+                        int dir, int shape, bool mixed) {
+  if (!ngroups) return;
+#define SG_FAC(T, DIR, S) k_line_factor<T, DIR, S><<<ngroups, kLineThreads, 0, s>>>(d, m, l)
+#define SG_FAC_S(T, DIR) \
+  do { if (shape == 0) SG_FAC(T, DIR, 16); else if (shape == 1) SG_FAC(T, DIR, 64); else SG_FAC(T, DIR, 256); } while (0)
+  if (mixed) { if (dir == 0) SG_FAC_S(float, 0); else SG_FAC_S(float, 1); }
+  else { if (dir == 0) SG_FAC_S(double, 0); else SG_FAC_S(double, 1); }
+#undef SG_FAC_S
+#undef SG_FAC
 }
 
 template <typename T, int S>

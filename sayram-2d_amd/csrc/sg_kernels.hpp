@@ -10,8 +10,8 @@ void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
-void launch_line_factor(hipStream_t s, const InstDev *d, const LineMap *m, int n, int l, int dir,
-                        bool mixed);
+void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
+                        int dir, int shape, bool mixed);
 // shape: tile steps 16, 64 or 256 (lines per group 64, 16, 4)
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
