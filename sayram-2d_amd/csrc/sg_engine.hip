@@ -121,28 +121,25 @@ struct Slot {
 
 struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
-  std::vector<LineMap> smap[kMaxLev][2][2], cxmap[kMaxLev], cymap[kMaxLev];
   std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
       gcy[kMaxLev][kNumTileShapes];
 };
 
 // Tile shape of a line solve by line length and direction: short lines in
 // 64-line x 16-step tiles, longer ones in 16 x 64, long rows in 4 x 256 (rows
-// stay coalesced along the line; columns keep >= 16 lines per tile so that
-// the accesses across the lines stay coalesced). It depends only on the
-// instance, so results do not depend on the batch.
+// stay coalesced along the line), long columns in 16 x 256 with 1024 threads
+// (columns keep 16 lines per tile so that the accesses across the lines stay
+// coalesced). It depends only on the instance, so results do not depend on
+// the batch.
 static int tile_shape(int n, int dir) {
   if (n <= 24) return 0;
-  if (dir == 1 || n <= 160) return 1;
-  return 2;
+  if (dir == 0) return n <= 160 ? 1 : 2;
+  return n <= 384 ? 1 : 3;
 }
-
 
 struct DevMaps {   // offsets into one device buffer
   const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
   int ncmap[kMaxLev], npmap[kMaxLev];
-  const LineMap *smap[kMaxLev][2][2], *cxmap[kMaxLev], *cymap[kMaxLev];
-  int nsmap[kMaxLev][2][2], ncx[kMaxLev], ncy[kMaxLev];
   const LineGrp *gmap[kMaxLev][2][2][kNumTileShapes], *gcx[kMaxLev][kNumTileShapes],
       *gcy[kMaxLev][kNumTileShapes];
   int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
@@ -285,7 +282,8 @@ int Batch::ensure_capacity(int cap) {
 static int check_grid(const sg_grid *g) {
   if (!g || !g->x_edges || !g->y_edges) return fail(SG_ERR_ARG, "grid: null pointer");
   if (g->nx < 1 || g->ny < 1) return fail(SG_ERR_ARG, "grid: nx and ny must be >= 1");
-  if ((long long)g->nx * g->ny > (1LL << 31) - 1) return fail(SG_ERR_ARG, "grid: too many cells");
+  if ((long long)g->nx * g->ny > kMaxCells)
+    return fail(SG_ERR_ARG, "grid: more than 2^28 cells in one instance");
   if (level_dims(g->nx, g->ny).size() > (size_t)kMaxLev)
     return fail(SG_ERR_ARG, "grid: too many multigrid levels");
   // Grid2D::validate, source/Grid2D.h:44-67
@@ -326,7 +324,7 @@ int Batch::upload_coeffs(Slot &s, const sg_coeffs *c) {
   double *G = h.data(), *Lxx = G + N, *Lxy = Lxx + N, *Lyy = Lxy + N, *it = Lyy + N;
   for (int i = 0; i < nx; ++i)
     for (int j = 0; j < ny; ++j) {
-      const size_t q = (size_t)i * ny + j, K = (size_t)j * nx + i;
+      const size_t q = (size_t)i * ny + j, K = cell_slot(i, j, nx);
       const double g = c->G[q];
       if (!std::isfinite(g) || !std::isfinite(c->Dxx[q]) || !std::isfinite(c->Dxy[q]) ||
           !std::isfinite(c->Dyy[q]))
@@ -425,7 +423,7 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     for (int j = 0; j < ny; ++j) {
       const double v = f0[(size_t)i * ny + j];
       if (!std::isfinite(v)) return undo(fail(SG_ERR_ARG, "f0: non-finite value"));
-      fk[(size_t)j * nx + i] = v;
+      fk[cell_slot(i, j, nx)] = v;
     }
   if (hipMemcpy(s.dev.f, fk.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "upload of f0 failed"));
@@ -467,21 +465,16 @@ int Batch::rebuild() {
         if (!coarsest) M.pmap[l].push_back({c, k0});
       }
       if (coarsest) {
-        if (ly == 1) {
-          M.cxmap[l].push_back({c, 0});
+        if (ly == 1)
           M.gcx[l][tile_shape(lx, 0)].push_back({c, 0, 1, 1});
-        } else {
-          M.cymap[l].push_back({c, 0});
+        else
           M.gcy[l][tile_shape(ly, 1)].push_back({c, 0, 1, 1});
-        }
       } else {
-        for (int j = 0; j < ly; ++j) M.smap[l][0][j & 1].push_back({c, j});
-        for (int i = 0; i < lx; ++i) M.smap[l][1][i & 1].push_back({c, i});
-        // groups of kTileElems / S lines of one colour
+        // groups of tile_lines(shape) lines of one colour
         for (int d = 0; d < 2; ++d) {
           const int nl = d == 0 ? ly : lx;          // number of lines
           const int sh = tile_shape(d == 0 ? lx : ly, d);
-          const int per = kTileElems / tile_steps(sh);
+          const int per = tile_lines(sh);
           for (int p = 0; p < 2; ++p)
             for (int f = p; f < nl; f += 2 * per) {
               const int cntl = std::min(per, (nl - f + 1) / 2);
@@ -491,14 +484,25 @@ int Batch::rebuild() {
       }
     }
   }
+  // Longest lines first: a group walks its lines tile by tile, so the groups
+  // of the longest lines are the critical path of a sweep and must start at
+  // once; the short groups of other instances fill the machine around them.
+  for (int l = 0; l < kMaxLev; ++l)
+    for (int sh = 0; sh < kNumTileShapes; ++sh)
+      for (int d = 0; d < 2; ++d)
+        for (int p = 0; p < 2; ++p) {
+          auto &v = M.gmap[l][d][p][sh];
+          auto len = [&](const LineGrp &g) {
+            return d == 0 ? table[g.inst].lev[l].nx : table[g.inst].lev[l].ny;
+          };
+          std::stable_sort(v.begin(), v.end(),
+                           [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
+        }
   // one device buffer for all maps
   size_t bytes = 0;
   auto sz = [&](size_t n, size_t e) { bytes += a256(n * e); };
   for (int l = 0; l < kMaxLev; ++l) {
     sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p) sz(M.smap[l][d][p].size(), sizeof(LineMap));
-    sz(M.cxmap[l].size(), sizeof(LineMap)); sz(M.cymap[l].size(), sizeof(LineMap));
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) sz(M.gmap[l][d][p][sh].size(), sizeof(LineGrp));
@@ -527,16 +531,6 @@ int Batch::rebuild() {
     dmn.ncmap[l] = (int)M.cmap[l].size();
     dmn.pmap[l] = (const BlkMap *)put(M.pmap[l].data(), M.pmap[l].size(), sizeof(BlkMap));
     dmn.npmap[l] = (int)M.pmap[l].size();
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p) {
-        dmn.smap[l][d][p] = (const LineMap *)put(M.smap[l][d][p].data(), M.smap[l][d][p].size(),
-                                                  sizeof(LineMap));
-        dmn.nsmap[l][d][p] = (int)M.smap[l][d][p].size();
-      }
-    dmn.cxmap[l] = (const LineMap *)put(M.cxmap[l].data(), M.cxmap[l].size(), sizeof(LineMap));
-    dmn.ncx[l] = (int)M.cxmap[l].size();
-    dmn.cymap[l] = (const LineMap *)put(M.cymap[l].data(), M.cymap[l].size(), sizeof(LineMap));
-    dmn.ncy[l] = (int)M.cymap[l].size();
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
@@ -640,8 +634,8 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   tend(3);
   if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
     tbegin(4);
-    launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid, bid);
-    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l);
+    if (!pre_zero) launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, xid, bid);
+    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l, pre_zero, xid, bid);
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
@@ -854,10 +848,18 @@ int Batch::debug_assemble(int slot, double *bands, double *R) {
   HIPCHK(hipGetLastError());
   const Slot &s = slots[slot];
   const size_t N = (size_t)s.nx * s.ny;
+  std::vector<double> h(6 * N);
   for (int b = 0; b < 5; ++b)
-    HIPCHK(hipMemcpyAsync(bands + b * N, s.dev.band64[b], N * 8, hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpyAsync(R, s.dev.R, N * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(h.data() + b * N, s.dev.band64[b], N * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(h.data() + 5 * N, s.dev.R, N * 8, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  // device cell order -> K = j*nx + i
+  for (int j = 0; j < s.ny; ++j)
+    for (int i = 0; i < s.nx; ++i) {
+      const size_t K = (size_t)j * s.nx + i, d = cell_slot(i, j, s.nx);
+      for (int b = 0; b < 5; ++b) bands[b * N + K] = h[b * N + d];
+      R[K] = h[5 * N + d];
+    }
   return SG_OK;
 }
 

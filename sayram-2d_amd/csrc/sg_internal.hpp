@@ -1,10 +1,20 @@
 // sg_internal.hpp -- device-side data structures of the gfx950 PPFV step.
 //
 // One instance (one Solver in the reference, source/Solver.h:18-82) lives in
-// one device slab. Cell arrays are stored in system order K = j*nx + i
-// (source/Mesh.h:66-68), so the 5-point operator is 5 coalesced bands and the
-// multigrid levels are 9 coalesced bands. Many instances of different sizes are
+// one device slab. The 5-point operator is 5 bands and the multigrid levels are
+// 9 bands, one value per cell each. Many instances of different sizes are
 // processed by the same launches through per-level block/line maps.
+//
+// Device cell order. The reference numbers cells K = j*nx + i
+// (source/Mesh.h:66-68). On the device every cell array of a level stores row
+// j as its even-i cells followed by its odd-i cells:
+//   slot(i, j) = j*nx + (i even ? i/2 : hx + i/2),   hx = (nx + 1) / 2.
+// The zebra column sweeps solve every other column; in this order the columns
+// of one colour are adjacent in memory, so their loads use whole cache lines
+// (in the plain order every fetched line is half wasted). Rows stay two
+// contiguous runs, and the coarse row J (the even points of fine row 2J) is
+// exactly the first half of the fine row. Host arrays keep the reference's
+// layouts; the conversion happens at upload and download.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -12,12 +22,33 @@
 namespace sg {
 
 constexpr int kMaxLev = 16;
+// Cells per instance: every per-cell array then stays below 2 GiB, so kernels
+// may address it with 32-bit byte offsets from its base.
+constexpr long long kMaxCells = 1LL << 28;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
-constexpr int kLineBlock = 64;  // lines per workgroup in line kernels (one wave)
-constexpr int kLineThreads = 256;   // threads of a line-solve workgroup
-constexpr int kTileElems = 1024;    // elements of one LDS tile (lines x steps), 4 per thread
-constexpr int kNumTileShapes = 3;   // tile steps S = 16, 64, 256 (lines 64, 16, 4)
-__host__ __device__ constexpr int tile_steps(int sc) { return sc == 0 ? 16 : sc == 1 ? 64 : 256; }
+// Tile shapes of the line solves: L lines x S steps staged through LDS by a
+// workgroup of NT threads, each thread owning 4 consecutive steps of a line in
+// the solve:
+//   0: 64 x 16 (256 threads)  short lines
+//   1: 16 x 64 (256 threads)  medium lines, and columns
+//   2:  4 x 256 (256 threads) long rows (a row is contiguous in memory)
+//   3: 16 x 256 (1024 threads) long columns (16 columns of one colour are 64
+//      contiguous bytes per row; 4x fewer serial tiles than shape 1)
+constexpr int kNumTileShapes = 4;
+__host__ __device__ constexpr int tile_steps(int sh) { return sh == 0 ? 16 : sh == 1 ? 64 : 256; }
+__host__ __device__ constexpr int tile_lines(int sh) {
+  return sh == 0 ? 64 : sh == 1 ? 16 : sh == 2 ? 4 : 16;
+}
+__host__ __device__ constexpr int tile_threads(int sh) { return sh == 3 ? 1024 : 256; }
+
+__host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
+// position of column i within its row
+__host__ __device__ inline int xslot(int i, int hx) { return (i & 1) ? hx + (i >> 1) : (i >> 1); }
+// column stored at position p of a row
+__host__ __device__ inline int xcol(int p, int hx) { return p < hx ? 2 * p : 2 * (p - hx) + 1; }
+__host__ __device__ inline size_t cell_slot(int i, int j, int nx) {
+  return (size_t)j * nx + xslot(i, half_x(nx));
+}
 
 // band slots; level 0 uses the first five (the operator is 5-point), coarse
 // Galerkin levels use all nine.
@@ -66,7 +97,7 @@ struct InstDev {
   const double *xe, *ye;
   const double *wx0, *wx1, *wy0, *wy1;
   const double *bcv[4];
-  // Equation inputs, K order: G, Lambda = G*D (source/Solver.cc:57-65), 1/tau
+  // Equation inputs, device cell order: G, Lambda = G*D (source/Solver.cc:57-65), 1/tau
   const double *G, *Lxx, *Lxy, *Lyy, *itau;
   // FP64 state
   double *f, *R, *x, *r64;
@@ -79,7 +110,6 @@ struct InstDev {
 };
 
 struct BlkMap { int inst; int k0; };      // cell-parallel: 256 cells of one instance
-struct LineMap { int inst; int line; };   // line-parallel: one row or column
 struct LineGrp { int inst; int first; int count; int step; };   // lines first + k*step, k < count
 
 }  // namespace sg

@@ -7,7 +7,9 @@
 //
 // Nothing here is a dense contraction, so MFMA is not used: every kernel is
 // bound by HBM bandwidth or by latency, and is written for coalesced FP64/FP32
-// streams along K = j*nx + i.
+// streams over the device cell order of sg_internal.hpp (rows split into their
+// even and odd columns). A cell kernel's work-item K is a storage slot; it
+// decodes (i, j) with cell_ij and finds its x-neighbours with xslot.
 #include "sg_kernels.hpp"
 
 #include <cfloat>
@@ -17,6 +19,25 @@ namespace sg {
 static constexpr double kEps = DBL_EPSILON;   // gEPS, source/common.h:38
 
 template <typename T> __device__ __forceinline__ T *ptr(void *p) { return static_cast<T *>(p); }
+
+// Element `slot` of an array whose base is uniform across the workgroup, through
+// a 32-bit byte offset: the address is base + zext(offset), the scalar-base form
+// of the global loads and stores, so one offset register serves every array of
+// the same element type. Instances hold at most kMaxCells cells, so the offsets
+// of FP64 arrays stay below 2^31.
+template <typename T> __device__ __forceinline__ const T &at32(const T *base, uint32_t slot) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      slot * (uint32_t)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ T &at32(T *base, uint32_t slot) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + slot * (uint32_t)sizeof(T));
+}
+
+// (i, j) of storage slot K of a level with nx columns (hx = half_x(nx))
+__device__ __forceinline__ void cell_ij(int K, int nx, int hx, int &i, int &j) {
+  j = K / nx;
+  i = xcol(K - j * nx, hx);
+}
 template <typename T> __device__ __forceinline__ const T *cptr(const void *p) {
   return static_cast<const T *>(p);
 }
@@ -90,7 +111,8 @@ __global__ void k_copy_counter(const int *__restrict__ src, int *__restrict__ ds
 // XMAX, YMIN, YMAX there, so the precedence here is the reverse.
 __device__ double vertex_value(const InstDev &I, const double *__restrict__ f, int vi, int vj) {
 #pragma clang fp contract(off)
-  const int nx = I.nx, ny = I.ny;
+  const int nx = I.nx, ny = I.ny, hx = half_x(nx);
+  auto F = [&](int ci, int cj) { return f[(size_t)cj * nx + xslot(ci, hx)]; };
   if (vj == ny && (I.bcmask & 8)) return I.bcv[3][vi];
   if (vj == 0 && (I.bcmask & 4)) return I.bcv[2][vi];
   if (vi == nx && (I.bcmask & 2)) return I.bcv[1][vj];
@@ -98,20 +120,19 @@ __device__ double vertex_value(const InstDev &I, const double *__restrict__ f, i
   const bool bx = (vi == 0 || vi == nx), by = (vj == 0 || vj == ny);
   if (bx && by) {   // corners copy the adjacent cell (:316-319)
     const int ci = vi == 0 ? 0 : nx - 1, cj = vj == 0 ? 0 : ny - 1;
-    return f[(size_t)cj * nx + ci];
+    return F(ci, cj);
   }
   if (bx) {         // vertical boundary edge: linear in y (:355-366)
     const int ci = vi == 0 ? 0 : nx - 1;
-    return I.wy0[vj] * f[(size_t)(vj - 1) * nx + ci] + I.wy1[vj] * f[(size_t)vj * nx + ci];
+    return I.wy0[vj] * F(ci, vj - 1) + I.wy1[vj] * F(ci, vj);
   }
   if (by) {         // horizontal boundary edge: linear in x (:371-381)
     const int cj = vj == 0 ? 0 : ny - 1;
-    return I.wx0[vi] * f[(size_t)cj * nx + vi - 1] + I.wx1[vi] * f[(size_t)cj * nx + vi];
+    return I.wx0[vi] * F(vi - 1, cj) + I.wx1[vi] * F(vi, cj);
   }
   const double wxL = I.wx0[vi], wxR = I.wx1[vi], wyB = I.wy0[vj], wyT = I.wy1[vj];
-  const size_t kb = (size_t)(vj - 1) * nx, kt = (size_t)vj * nx;
-  return wxL * wyB * f[kb + vi - 1] + wxR * wyB * f[kb + vi] + wxL * wyT * f[kt + vi - 1] +
-         wxR * wyT * f[kt + vi];
+  return wxL * wyB * F(vi - 1, vj - 1) + wxR * wyB * F(vi, vj - 1) + wxL * wyT * F(vi - 1, vj) +
+         wxR * wyT * F(vi, vj);
 }
 
 // a_sigma_func (source/Solver.cc:68-97) with the edge table of
@@ -200,7 +221,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
   const int nx = I.nx, ny = I.ny;
   const int K = m.k0 + threadIdx.x;
   if (K >= nx * ny) return;
-  const int j = K / nx, i = K - j * nx;
+  const int hx = half_x(nx);
+  int i, j;
+  cell_ij(K, nx, hx, i, j);
+  const size_t kw = (size_t)j * nx + xslot(i - 1, hx), ke = (size_t)j * nx + xslot(i + 1, hx);
   const double *__restrict__ f = I.f;
   const double *xe = I.xe, *ye = I.ye;
   const double vSW = vertex_value(I, f, i, j), vSE = vertex_value(I, f, i + 1, j);
@@ -209,14 +233,14 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
   const double fK = f[K];
   double d = 0.0, R = 0.0, bw = 0.0, be = 0.0, bs = 0.0, bn = 0.0, AK, AL;
   if (i > 0) {   // own west face (west-pair loop, :173-175)
-    const double LL[3] = {I.Lxx[K - 1], I.Lxy[K - 1], I.Lyy[K - 1]};
-    face_pair(xe, ye, i, j, 0, LK, LL, fK, f[K - 1], vNW, vSW, AK, AL);
+    const double LL[3] = {I.Lxx[kw], I.Lxy[kw], I.Lyy[kw]};
+    face_pair(xe, ye, i, j, 0, LK, LL, fK, f[kw], vNW, vSW, AK, AL);
     d += AK;
     bw = 0.0 + -AL;
   }
   if (i < nx - 1) {   // east face, owned by (i+1, j): this cell is its L
-    const double LO[3] = {I.Lxx[K + 1], I.Lxy[K + 1], I.Lyy[K + 1]};
-    face_pair(xe, ye, i + 1, j, 0, LO, LK, f[K + 1], fK, vNE, vSE, AK, AL);
+    const double LO[3] = {I.Lxx[ke], I.Lxy[ke], I.Lyy[ke]};
+    face_pair(xe, ye, i + 1, j, 0, LO, LK, f[ke], fK, vNE, vSE, AK, AL);
     d += AL;
     be = 0.0 + -AK;
   }
@@ -333,7 +357,9 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
   const int nx = L.nx, ny = L.ny;
   const int K = m.k0 + threadIdx.x;
   if (K >= nx * ny) return;
-  const int j = K / nx, i = K - j * nx;
+  const int hx = half_x(nx);
+  int i, j;
+  cell_ij(K, nx, hx, i, j);
   double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
   const bool io = i & 1, jo = j & 1;
   if (!io && !jo) {
@@ -345,8 +371,8 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
   } else {
     const St9 a = load_st<NB>(L, K);
     double sW = 0, nW = 0, sE = 0, nE = 0, wS = 0, eS = 0, wN = 0, eN = 0;
-    w_yedge<NB>(L, K - 1, j, sW, nW);
-    if (i + 1 < nx) w_yedge<NB>(L, K + 1, j, sE, nE);
+    w_yedge<NB>(L, (size_t)j * nx + xslot(i - 1, hx), j, sW, nW);
+    if (i + 1 < nx) w_yedge<NB>(L, (size_t)j * nx + xslot(i + 1, hx), j, sE, nE);
     w_xedge<NB>(L, K - nx, i, wS, eS);
     if (j + 1 < ny) w_xedge<NB>(L, K + nx, i, wN, eN);
     const double c = a.c;
@@ -376,7 +402,9 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   const int cx = Cl.nx, cy = Cl.ny, fx = F.nx, fy = F.ny;
   const int KC = m.k0 + threadIdx.x;
   if (KC >= cx * cy) return;
-  const int J = KC / cx, Ic = KC - J * cx;
+  const int hxf = half_x(fx);
+  int Ic, J;
+  cell_ij(KC, cx, half_x(cx), Ic, J);
   const T *__restrict__ pw[4] = {cptr<T>(F.pw[0]), cptr<T>(F.pw[1]), cptr<T>(F.pw[2]),
                                  cptr<T>(F.pw[3])};
   constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
@@ -390,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
     for (int fa = -1; fa <= 1; ++fa) {
       const int fi = 2 * Ic + fa;
       if (fi < 0 || fi >= fx) continue;
-      const size_t kf = (size_t)fj * fx + fi;
+      const size_t kf = (size_t)fj * fx + xslot(fi, hxf);
       // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
       const int slot = (fa < 0 ? 1 : 0) + 2 * (fb < 0 ? 1 : 0);
       const double wf = (double)pw[slot][kf];
@@ -400,7 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
         const int gi = 2 * Ic + ga, gj = 2 * J + gb;
         if (gi < 0 || gi >= fx || gj < 0 || gj >= fy) continue;
         const double wa = wf * F.band64[b][kf];
-        const size_t kg = (size_t)gj * fx + gi;
+        const size_t kg = (size_t)gj * fx + xslot(gi, hxf);
         const int oi = (ga >= 0 ? ga / 2 : -((1 - ga) / 2)) + 1;   // floor(ga / 2) + 1
         const int oj = (gb >= 0 ? gb / 2 : -((1 - gb) / 2)) + 1;
 #pragma unroll
@@ -441,13 +469,14 @@ __device__ __forceinline__ void level_xb(const InstDev &I, int l, int xid, int b
 // a line compose their 4 steps' matrices (rescaled to stay in range), scan
 // them across the line and carry (P, Q) between tiles; each thread then runs
 // the exact recurrence over its 4 steps from the value the scan gives it.
-template <typename T, int DIR, int S>
-__global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__restrict__ insts,
-                                                             const LineGrp *__restrict__ map,
-                                                             int l) {
-  constexpr int L = kTileElems / S;
-  constexpr int W = S / 4;
-  constexpr int Q = kTileElems / kLineThreads;
+template <typename T, int DIR, int SH>
+__global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev *__restrict__ insts,
+                                                                  const LineGrp *__restrict__ map,
+                                                                  int l) {
+  constexpr int S = tile_steps(SH), L = tile_lines(SH), NT = tile_threads(SH);
+  constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
+  constexpr int Q = S * L / NT;      // elements per thread in the loads
+  static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
   __shared__ double s_lo[L][S + 1];
   __shared__ double s_d[L][S + 1];
   __shared__ double s_up[L][S + 1];
@@ -463,16 +492,20 @@ __global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__r
   T *__restrict__ inv = ptr<T>(Lv.linv[DIR]);
   T *__restrict__ cu = ptr<T>(Lv.lup[DIR]);
   const int n = DIR == 0 ? nx : ny;
-  const size_t st = DIR == 0 ? 1 : (size_t)nx;
-  const size_t so = DIR == 0 ? (size_t)nx : 1;
+  const int hx = half_x(nx);
+  // storage slot of step t of line li (rows: DIR 0, columns: DIR 1)
+  auto at = [&](int li, int t) -> uint32_t {
+    return DIR == 0 ? (uint32_t)(li * nx + xslot(t, hx)) : (uint32_t)(t * nx + xslot(li, hx));
+  };
   const int cnt = gr.count;
   const int tid = threadIdx.x;
   int ek[Q], eu[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int e = q * kLineThreads + tid;
+    const int e = q * NT + tid;
+    const int v = e % S;
     ek[q] = DIR == 0 ? e / S : e % L;
-    eu[q] = DIR == 0 ? e % S : e / L;
+    eu[q] = DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
   }
   const int sk = tid / W, sc = tid % W;
   const bool sk_ok = sk < cnt;
@@ -486,10 +519,10 @@ __global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__r
       vl[q] = 0.0; vd[q] = 1.0; vu[q] = 0.0;
       if (k < cnt && u < tw) {
         const int t = t0 + u;
-        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)t * st;
-        vl[q] = t > 0 ? lo[K] : 0.0;
-        vd[q] = bc[K];
-        vu[q] = t < n - 1 ? up[K] : 0.0;
+        const uint32_t K = at(gr.first + k * gr.step, t);
+        vl[q] = t > 0 ? at32(lo, K) : 0.0;
+        vd[q] = at32(bc, K);
+        vu[q] = t < n - 1 ? at32(up, K) : 0.0;
       }
     }
 #pragma unroll
@@ -555,9 +588,9 @@ __global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__r
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       if (k < cnt && u < tw) {
-        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st;
-        inv[K] = (T)s_d[k][u];
-        cu[K] = (T)s_up[k][u];
+        const uint32_t K = at(gr.first + k * gr.step, t0 + u);
+        at32(inv, K) = (T)s_d[k][u];
+        at32(cu, K) = (T)s_up[k][u];
       }
     }
     __syncthreads();
@@ -578,13 +611,14 @@ __global__ void __launch_bounds__(kLineThreads) k_line_factor(const InstDev *__r
 // eliminated right-hand side in x (no other line of this colour reads it);
 // the backward pass turns it into the solution. ZERO: the other lines are
 // zero (first sweep of a cycle).
-template <typename T, int NB, int DIR, bool ZERO, int S>
-__global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict__ insts,
-                                                      const LineGrp *__restrict__ map, int l,
-                                                      int xid, int bid) {
-  constexpr int L = kTileElems / S;          // lines per group
-  constexpr int W = S / 4;                   // threads per line in the solve
-  constexpr int Q = kTileElems / kLineThreads;
+template <typename T, int NB, int DIR, bool ZERO, int SH>
+__global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__restrict__ insts,
+                                                           const LineGrp *__restrict__ map, int l,
+                                                           int xid, int bid) {
+  constexpr int S = tile_steps(SH), L = tile_lines(SH), NT = tile_threads(SH);
+  constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
+  constexpr int Q = S * L / NT;      // elements per thread in the loads
+  static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
   __shared__ T s_r[L][S + 1];
   __shared__ T s_a[L][S + 1];
   __shared__ T s_i[L][S + 1];
@@ -610,17 +644,21 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
   const T *__restrict__ cu = cptr<T>(Lv.lup[DIR]);
   const int n = DIR == 0 ? nx : ny;            // line length
   const int nperp = DIR == 0 ? ny : nx;
-  const size_t st = DIR == 0 ? 1 : (size_t)nx;  // along the line
-  const size_t so = DIR == 0 ? (size_t)nx : 1;  // to the neighbouring lines
+  const int hx = half_x(nx);
+  // storage slot of step t of line li (rows: DIR 0, columns: DIR 1)
+  auto at = [&](int li, int t) -> uint32_t {
+    return DIR == 0 ? (uint32_t)(li * nx + xslot(t, hx)) : (uint32_t)(t * nx + xslot(li, hx));
+  };
   const int cnt = gr.count;
   const int tid = threadIdx.x;
   // load/store layout: element e -> (line k, step u), coalesced in memory
   int ek[Q], eu[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int e = q * kLineThreads + tid;
+    const int e = q * NT + tid;
+    const int v = e % S;
     ek[q] = DIR == 0 ? e / S : e % L;
-    eu[q] = DIR == 0 ? e % S : e / L;
+    eu[q] = DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
   }
   // solve layout: thread -> (line sk, steps 4*sc .. 4*sc+3)
   const int sk = tid / W, sc = tid % W;
@@ -637,23 +675,23 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
       dmm[q] = dmp[q] = dpm[q] = dpp[q] = xmm[q] = xmp[q] = xpm[q] = xpp[q] = T(0);
       if (k < cnt && u < tw) {
         const int li = gr.first + k * gr.step, t = t0 + u;
-        const size_t K = (size_t)li * so + (size_t)t * st;
-        vb[q] = b[K];
-        vlo[q] = t > 0 ? lo[K] : T(0);
-        viv[q] = inv[K];
+        const uint32_t K = at(li, t);
+        vb[q] = at32(b, K);
+        vlo[q] = t > 0 ? at32(lo, K) : T(0);
+        viv[q] = at32(inv, K);
         if (!ZERO) {
           if (li > 0) {
-            vom[q] = om[K]; xm[q] = x[K - so];
+            vom[q] = at32(om, K); xm[q] = at32(x, at(li - 1, t));
             if (NB == 9) {
-              if (t > 0) { dmm[q] = omm[K]; xmm[q] = x[K - so - st]; }
-              if (t < n - 1) { dmp[q] = omp[K]; xmp[q] = x[K - so + st]; }
+              if (t > 0) { dmm[q] = at32(omm, K); xmm[q] = at32(x, at(li - 1, t - 1)); }
+              if (t < n - 1) { dmp[q] = at32(omp, K); xmp[q] = at32(x, at(li - 1, t + 1)); }
             }
           }
           if (li < nperp - 1) {
-            vop[q] = op[K]; xp[q] = x[K + so];
+            vop[q] = at32(op, K); xp[q] = at32(x, at(li + 1, t));
             if (NB == 9) {
-              if (t > 0) { dpm[q] = opm[K]; xpm[q] = x[K + so - st]; }
-              if (t < n - 1) { dpp[q] = opp[K]; xpp[q] = x[K + so + st]; }
+              if (t > 0) { dpm[q] = at32(opm, K); xpm[q] = at32(x, at(li + 1, t - 1)); }
+              if (t < n - 1) { dpp[q] = at32(opp, K); xpp[q] = at32(x, at(li + 1, t + 1)); }
             }
           }
         }
@@ -704,7 +742,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       if (k < cnt && u < tw)
-        x[(size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st] = s_r[k][u];
+        at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
     }
     __syncthreads();
   }
@@ -719,9 +757,9 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
       const int k = ek[q], u = eu[q];
       vd[q] = vc[q] = T(0);
       if (k < cnt && u < tw) {
-        const size_t K = (size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st;
-        vd[q] = x[K];
-        vc[q] = cu[K];
+        const uint32_t K = at(gr.first + k * gr.step, t0 + u);
+        vd[q] = at32(x, K);
+        vc[q] = at32(cu, K);
       }
     }
 #pragma unroll
@@ -766,38 +804,52 @@ __global__ void __launch_bounds__(kLineThreads) k_line(const InstDev *__restrict
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       if (k < cnt && u < tw)
-        x[(size_t)(gr.first + k * gr.step) * so + (size_t)(t0 + u) * st] = s_r[k][u];
+        at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
     }
     __syncthreads();
   }
 }
 
+// (A x)(i, j) at slot K; kw, ke are the slots of (i-1, j) and (i+1, j)
 template <typename T, int NB>
 __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j, const T *x) {
-  const int nx = L.nx, ny = L.ny;
+  const int nx = L.nx, ny = L.ny, hx = half_x(nx);
+  const size_t kw = (size_t)j * nx + xslot(i - 1, hx), ke = (size_t)j * nx + xslot(i + 1, hx);
   T s = cptr<T>(L.band[BC])[K] * x[K];
-  if (i > 0) s += cptr<T>(L.band[BW])[K] * x[K - 1];
-  if (i < nx - 1) s += cptr<T>(L.band[BE])[K] * x[K + 1];
+  if (i > 0) s += cptr<T>(L.band[BW])[K] * x[kw];
+  if (i < nx - 1) s += cptr<T>(L.band[BE])[K] * x[ke];
   if (j > 0) {
     s += cptr<T>(L.band[BS])[K] * x[K - nx];
     if (NB == 9) {
-      if (i > 0) s += cptr<T>(L.band[BSW])[K] * x[K - nx - 1];
-      if (i < nx - 1) s += cptr<T>(L.band[BSE])[K] * x[K - nx + 1];
+      if (i > 0) s += cptr<T>(L.band[BSW])[K] * x[kw - nx];
+      if (i < nx - 1) s += cptr<T>(L.band[BSE])[K] * x[ke - nx];
     }
   }
   if (j < ny - 1) {
     s += cptr<T>(L.band[BN])[K] * x[K + nx];
     if (NB == 9) {
-      if (i > 0) s += cptr<T>(L.band[BNW])[K] * x[K + nx - 1];
-      if (i < nx - 1) s += cptr<T>(L.band[BNE])[K] * x[K + nx + 1];
+      if (i > 0) s += cptr<T>(L.band[BNW])[K] * x[kw + nx];
+      if (i < nx - 1) s += cptr<T>(L.band[BNE])[K] * x[ke + nx];
     }
   }
   return s;
 }
 
+// M x at slot K with the FP64 level-0 bands
+__device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const double *x) {
+  const int nx = I.nx, hx = half_x(nx);
+  double s = I.band64[BC][K] * x[K];
+  if (i > 0) s += I.band64[BW][K] * x[(size_t)j * nx + xslot(i - 1, hx)];
+  if (i < nx - 1) s += I.band64[BE][K] * x[(size_t)j * nx + xslot(i + 1, hx)];
+  if (j > 0) s += I.band64[BS][K] * x[K - nx];
+  if (j < I.ny - 1) s += I.band64[BN][K] * x[K + nx];
+  return s;
+}
+
 // r_l = b_l - A_l x_l into the level's scratch (the line-solve buffer cp is
-// free between the pre- and post-smoothing sweeps). ZERO: x_l = 0.
-template <typename T, int NB, bool ZERO>
+// free between the pre- and post-smoothing sweeps). Only needed after a
+// pre-smoothing sweep: with x_l = 0 the restriction reads b_l itself.
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
@@ -807,16 +859,19 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
   const Lev &F = I.lev[l];
   const int K = m.k0 + threadIdx.x;
   if (K >= F.nx * F.ny) return;
-  const int j = K / F.nx, i = K - j * F.nx;
+  int i, j;
+  cell_ij(K, F.nx, half_x(F.nx), i, j);
   T *x; const T *b;
   level_xb<T>(I, l, xid, bid, x, b);
-  ptr<T>(F.cp)[K] = ZERO ? b[K] : b[K] - stencil_apply<T, NB>(F, K, i, j, x);
+  ptr<T>(F.cp)[K] = b[K] - stencil_apply<T, NB>(F, K, i, j, x);
 }
 
-// b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J)
-template <typename T>
+// b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J).
+// FROM_B: x_l is still zero, so r_l is the level's right-hand side itself.
+template <typename T, bool FROM_B>
 __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
-                                                    const BlkMap *__restrict__ map, int l) {
+                                                    const BlkMap *__restrict__ map, int l,
+                                                    int xid, int bid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (I.sc->status != ST_ACTIVE) return;
@@ -824,8 +879,12 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   const Lev &Cl = I.lev[l + 1];
   const int KC = m.k0 + threadIdx.x;
   if (KC >= Cl.nx * Cl.ny) return;
-  const int J = KC / Cl.nx, Ic = KC - J * Cl.nx;
-  const T *__restrict__ r = cptr<T>(F.cp);
+  int Ic, J;
+  cell_ij(KC, Cl.nx, half_x(Cl.nx), Ic, J);
+  const int hxf = half_x(F.nx);
+  T *x; const T *b;
+  level_xb<T>(I, l, xid, bid, x, b);
+  const T *__restrict__ r = FROM_B ? b : cptr<T>(F.cp);
   T s = T(0);
   for (int fj = 2 * J - 1; fj <= 2 * J + 1; ++fj) {
     if (fj < 0 || fj >= F.ny) continue;
@@ -833,7 +892,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
     for (int fi = 2 * Ic - 1; fi <= 2 * Ic + 1; ++fi) {
       if (fi < 0 || fi >= F.nx) continue;
       const int slot = (Ic - (fi >> 1)) + 2 * dJ;
-      const size_t kf = (size_t)fj * F.nx + fi;
+      const size_t kf = (size_t)fj * F.nx + xslot(fi, hxf);
       s += cptr<T>(F.pw[slot])[kf] * r[kf];
     }
   }
@@ -852,14 +911,17 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
   const Lev &Cl = I.lev[l + 1];
   const int K = m.k0 + threadIdx.x;
   if (K >= F.nx * F.ny) return;
-  const int j = K / F.nx, i = K - j * F.nx;
+  int i, j;
+  cell_ij(K, F.nx, half_x(F.nx), i, j);
   const int I0 = i >> 1, J0 = j >> 1;
   const int I1 = I0 + 1 < Cl.nx ? I0 + 1 : I0, J1 = J0 + 1 < Cl.ny ? J0 + 1 : J0;
   const T *__restrict__ xc = cptr<T>(Cl.x);
+  const int hxc = half_x(Cl.nx);
   const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
+  const int s0 = xslot(I0, hxc), s1 = xslot(I1, hxc);
   // weights of parents outside the coarse grid are zero
-  const T e = cptr<T>(F.pw[0])[K] * xc[r0 + I0] + cptr<T>(F.pw[1])[K] * xc[r0 + I1] +
-              cptr<T>(F.pw[2])[K] * xc[r1 + I0] + cptr<T>(F.pw[3])[K] * xc[r1 + I1];
+  const T e = cptr<T>(F.pw[0])[K] * xc[r0 + s0] + cptr<T>(F.pw[1])[K] * xc[r0 + s1] +
+              cptr<T>(F.pw[2])[K] * xc[r1 + s0] + cptr<T>(F.pw[3])[K] * xc[r1 + s1];
   T *x = l == 0 ? ptr<T>(I.kv[xid]) : ptr<T>(F.x);
   x[K] = ZERO ? e : x[K] + e;
 }
@@ -899,7 +961,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
   const int N = I.nx * I.ny;
   double v[2] = {0.0, 0.0};
   if (K < N) {
-    const int j = K / I.nx, i = K - j * I.nx;
+    int i, j;
+    cell_ij(K, I.nx, half_x(I.nx), i, j);
     const double r = I.R[K] - spmv5<double>(I, K, i, j, I.x);
     ptr<double>(I.kv[VR])[K] = r;
     ptr<double>(I.kv[VRH])[K] = r;
@@ -936,7 +999,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
   const int N = I.nx * I.ny;
   T acc[1] = {T(0)};
   if (K < N) {
-    const int j = K / I.nx, i = K - j * I.nx;
+    int i, j;
+    cell_ij(K, I.nx, half_x(I.nx), i, j);
     const T v = spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
     ptr<T>(I.kv[VV])[K] = v;
     acc[0] = cptr<T>(I.kv[VRH])[K] * v;
@@ -973,7 +1037,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
   const int N = I.nx * I.ny;
   T acc[2] = {T(0), T(0)};
   if (K < N) {
-    const int j = K / I.nx, i = K - j * I.nx;
+    int i, j;
+    cell_ij(K, I.nx, half_x(I.nx), i, j);
     const T t = spmv5<T>(I, K, i, j, cptr<T>(I.kv[VSH]));
     ptr<T>(I.kv[VT])[K] = t;
     acc[0] = t * cptr<T>(I.kv[VS])[K];
@@ -1021,13 +1086,9 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restric
   const int N = I.nx * I.ny;
   double v[2] = {0.0, 0.0};
   if (K < N) {
-    const int j = K / I.nx, i = K - j * I.nx;
-    double s = I.band64[BC][K] * I.x[K];
-    if (i > 0) s += I.band64[BW][K] * I.x[K - 1];
-    if (i < I.nx - 1) s += I.band64[BE][K] * I.x[K + 1];
-    if (j > 0) s += I.band64[BS][K] * I.x[K - I.nx];
-    if (j < I.ny - 1) s += I.band64[BN][K] * I.x[K + I.nx];
-    const double r = I.R[K] - s;
+    int i, j;
+    cell_ij(K, I.nx, half_x(I.nx), i, j);
+    const double r = I.R[K] - spmv64(I, K, i, j, I.x);
     I.r64[K] = r;
     v[0] = r * r;
     v[1] = I.R[K] * I.R[K];
@@ -1079,14 +1140,10 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
   double v[3] = {0.0, 0.0, 0.0};
   double mn = DBL_MAX;
   if (K < N) {
-    const int j = K / I.nx, i = K - j * I.nx;
+    int i, j;
+    cell_ij(K, I.nx, half_x(I.nx), i, j);
     const double xv = I.x[K];
-    double s = I.band64[BC][K] * xv;
-    if (i > 0) s += I.band64[BW][K] * I.x[K - 1];
-    if (i < I.nx - 1) s += I.band64[BE][K] * I.x[K + 1];
-    if (j > 0) s += I.band64[BS][K] * I.x[K - I.nx];
-    if (j < I.ny - 1) s += I.band64[BN][K] * I.x[K + I.nx];
-    const double r = I.R[K] - s;
+    const double r = I.R[K] - spmv64(I, K, i, j, I.x);
     v[0] = r * r;
     v[1] = I.R[K] * I.R[K];
     v[2] = xv < 0.0 ? 1.0 : 0.0;
@@ -1255,12 +1312,13 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
   }
 }
 
-// (nx,ny) i-major <-> K order (j*nx+i) transposes for the host boundary
+// (nx,ny) i-major (the reference's Xtensor2d) <-> device cell order, for the
+// host boundary
 __global__ void __launch_bounds__(kBlock) k_to_korder(const double *__restrict__ src,
                                                      double *__restrict__ dst, int nx, int ny) {
   const long long K = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (K >= (long long)nx * ny) return;
-  const int j = (int)(K / nx), i = (int)(K - (long long)j * nx);
+  const int j = (int)(K / nx), i = xcol((int)(K - (long long)j * nx), half_x(nx));
   dst[K] = src[(size_t)i * ny + j];
 }
 __global__ void __launch_bounds__(kBlock) k_to_imajor(const double *__restrict__ src,
@@ -1268,7 +1326,7 @@ __global__ void __launch_bounds__(kBlock) k_to_imajor(const double *__restrict__
   const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (q >= (long long)nx * ny) return;
   const int i = (int)(q / ny), j = (int)(q - (long long)i * ny);
-  dst[q] = src[(size_t)j * nx + i];
+  dst[q] = src[cell_slot(i, j, nx)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1313,20 +1371,26 @@ void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l,
 void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
                         int dir, int shape, bool mixed) {
   if (!ngroups) return;
-#define SG_FAC(T, DIR, S) k_line_factor<T, DIR, S><<<ngroups, kLineThreads, 0, s>>>(d, m, l)
-#define SG_FAC_S(T, DIR) \
-  do { if (shape == 0) SG_FAC(T, DIR, 16); else if (shape == 1) SG_FAC(T, DIR, 64); else SG_FAC(T, DIR, 256); } while (0)
+#define SG_FAC(T, DIR, SH) k_line_factor<T, DIR, SH><<<ngroups, tile_threads(SH), 0, s>>>(d, m, l)
+#define SG_FAC_S(T, DIR)                                   \
+  do {                                                     \
+    if (shape == 0) SG_FAC(T, DIR, 0);                     \
+    else if (shape == 1) SG_FAC(T, DIR, 1);                \
+    else if (shape == 2) SG_FAC(T, DIR, 2);                \
+    else SG_FAC(T, DIR, 3);                                \
+  } while (0)
   if (mixed) { if (dir == 0) SG_FAC_S(float, 0); else SG_FAC_S(float, 1); }
   else { if (dir == 0) SG_FAC_S(double, 0); else SG_FAC_S(double, 1); }
 #undef SG_FAC_S
 #undef SG_FAC
 }
 
-template <typename T, int S>
+template <typename T, int SH>
 static void launch_line_s(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
                           int dir, bool zero, int xid, int bid) {
   const bool nine = l > 0;
-#define SG_LINE(NB, DIR, Z) k_line<T, NB, DIR, Z, S><<<ngroups, kLineThreads, 0, s>>>(d, m, l, xid, bid)
+#define SG_LINE(NB, DIR, Z) \
+  k_line<T, NB, DIR, Z, SH><<<ngroups, tile_threads(SH), 0, s>>>(d, m, l, xid, bid)
   if (nine) {
     if (dir == 0) { if (zero) SG_LINE(9, 0, true); else SG_LINE(9, 0, false); }
     else { if (zero) SG_LINE(9, 1, true); else SG_LINE(9, 1, false); }
@@ -1341,9 +1405,10 @@ template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  int shape, bool zero, int xid, int bid) {
   if (!ngroups) return;
-  if (shape == 0) launch_line_s<T, 16>(s, d, m, ngroups, l, dir, zero, xid, bid);
-  else if (shape == 1) launch_line_s<T, 64>(s, d, m, ngroups, l, dir, zero, xid, bid);
-  else launch_line_s<T, 256>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  if (shape == 0) launch_line_s<T, 0>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else if (shape == 1) launch_line_s<T, 1>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else if (shape == 2) launch_line_s<T, 2>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else launch_line_s<T, 3>(s, d, m, ngroups, l, dir, zero, xid, bid);
 }
 template void launch_line<double>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
                                   int, bool, int, int);
@@ -1351,28 +1416,28 @@ template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, 
                                  int, bool, int, int);
 
 template <typename T>
-void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
-                     int xid, int bid) {
+void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
+                     int bid) {
   if (!nb) return;
-  if (l == 0) {
-    if (zero) k_residual<T, 5, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_residual<T, 5, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  } else {
-    if (zero) k_residual<T, 9, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_residual<T, 9, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  }
+  if (l == 0) k_residual<T, 5><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_residual<T, 9><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
 }
-template void launch_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                      int, int);
-template void launch_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                     int, int);
+template void launch_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
+                                      int);
+template void launch_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
+                                     int);
 
 template <typename T>
-void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l) {
-  if (nb) k_restrict<T><<<nb, kBlock, 0, s>>>(d, m, l);
+void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
+                     int xid, int bid) {
+  if (!nb) return;
+  if (from_b) k_restrict<T, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_restrict<T, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
 }
-template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int);
-template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int);
+template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
+                                      int, int);
+template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
+                                     int, int);
 
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,

@@ -12,15 +12,17 @@ void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, i
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
                         int dir, int shape, bool mixed);
-// shape: tile steps 16, 64 or 256 (lines per group 64, 16, 4)
+// shape: tile shape 0..3 of sg_internal.hpp (lines x steps per workgroup)
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  int shape, bool zero, int xid, int bid);
 template <typename T>
-void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
-                     int xid, int bid);
+void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
+                     int bid);
+// from_b: restrict the level's right-hand side (x_l = 0) instead of its residual
 template <typename T>
-void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l);
+void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
+                     int xid, int bid);
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid);
