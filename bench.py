@@ -70,8 +70,8 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_mixed_batched(probs, warmup, steps):
-    b = sg.Batch(sg.make_opts(precision=sg.SG_MIXED))
+def run_batched(probs, warmup, steps, precision=sg.SG_MIXED):
+    b = sg.Batch(sg.make_opts(precision=precision))
     slots = [b.add(p) for p in probs]
     b.step(warmup)
     b.time_kernel("line_l0")
@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--big", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--precision", choices=["mixed", "fp64", "pc32"], default="mixed",
+                    help="precision of the batched step that is measured")
     ap.add_argument("--oracle-checks", type=int, default=3)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64,
@@ -200,13 +202,14 @@ def main():
     log(f"[bench] {len(probs)} instances, {cells / 1e6:.2f} M cells, inputs in "
         f"{time.perf_counter() - t0:.1f}s")
 
-    dt, (kms, kl, kbytes), st = run_mixed_batched(probs, args.warmup, args.steps)
+    prec = {"mixed": sg.SG_MIXED, "fp64": sg.SG_FP64, "pc32": sg.SG_PC32}[args.precision]
+    dt, (kms, kl, kbytes), st = run_batched(probs, args.warmup, args.steps, prec)
     value = cells * args.steps / dt
     iters = [s["last_iters"] for s in st]
     refs = [s["last_refinements"] for s in st]
-    log(f"[bench] mixed batched: {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} cell-steps/s, "
-        f"inner iterations mean {np.mean(iters):.1f} max {max(iters)}, refinements mean "
-        f"{np.mean(refs):.2f}")
+    log(f"[bench] {args.precision} batched: {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} "
+        f"cell-steps/s, Krylov iterations mean {np.mean(iters):.1f} max {max(iters)}, "
+        f"refinements mean {np.mean(refs):.2f}")
     if args.profile_only:
         return
 

@@ -44,7 +44,15 @@ typedef enum {
 
 typedef enum { SG_DIRICHLET = 0, SG_ZEROFLUX = 1 } sg_bc_type;   /* source/BCTypes.h:16 */
 typedef enum { SG_XMIN = 0, SG_XMAX = 1, SG_YMIN = 2, SG_YMAX = 3 } sg_side;
-typedef enum { SG_FP64 = 0, SG_MIXED = 1 } sg_precision;
+/* Precision of the implicit solve. Every mode assembles the operator, forms
+ * the true residual R - M f and advances f in FP64 (source/Solver.cc:270-290).
+ *  SG_FP64  FP64 BiCGSTAB preconditioned by an FP64 multigrid V-cycle.
+ *  SG_MIXED FP32 BiCGSTAB with an FP32 V-cycle inside FP64 iterative
+ *           refinement (the FP64 residual is corrected to rtol).
+ *  SG_PC32  FP64 BiCGSTAB preconditioned by an FP32 V-cycle: the Krylov
+ *           vectors, the operator and the convergence test stay FP64; only the
+ *           preconditioner, most of the traffic, runs in FP32. */
+typedef enum { SG_FP64 = 0, SG_MIXED = 1, SG_PC32 = 2 } sg_precision;
 
 /* Grid2D (source/Grid2D.h:16-68): strictly increasing cell edges. */
 typedef struct {
@@ -70,13 +78,14 @@ typedef struct {
 typedef struct {
   int precision;        /* sg_precision; default SG_FP64 */
   double rtol;          /* stop when ||R - M f||_2 <= rtol ||R||_2; default 1e-14 */
-  int max_iters;        /* BiCGSTAB iteration cap per step (FP64) or refinement
-                           sweeps (MIXED); default 200 / 12 */
+  int max_iters;        /* BiCGSTAB iteration cap per step (FP64, PC32) or
+                           refinement sweeps (MIXED); default 200 / 12 */
   double inner_rtol;    /* MIXED: FP32 inner solve tolerance; default 1e-5 */
   int max_inner;        /* MIXED: FP32 inner iteration cap; default 40 */
   int check_every;      /* iterations between host convergence checks; default 2 */
   int smoother;         /* bit0: pre x-lines, bit1: pre y-lines, bit2: post x,
-                           bit3: post y; default 15 (all) */
+                           bit3: post y; default 12 (post-smoothing, x-lines
+                           then y-lines) */
   int use_graph;        /* capture iteration chunks into hipGraphs; default 1 */
   int reserved[6];
 } sg_opts;

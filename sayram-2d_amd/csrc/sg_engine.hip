@@ -49,7 +49,7 @@ static std::vector<Dims> level_dims(int nx, int ny) {
 
 // Assigns every device pointer of an instance inside one slab starting at
 // `base` (0 to size it); returns the slab size in bytes.
-static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
+static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   size_t off = 0;
   auto take = [&](size_t bytes) -> void * {
     void *p = reinterpret_cast<void *>(base + off);
@@ -58,11 +58,14 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
   };
   const size_t N = (size_t)nx * ny;
   const size_t D = sizeof(double);
-  const size_t T = mixed ? sizeof(float) : sizeof(double);
+  const bool mixed = prec == SG_MIXED;    // FP32 Krylov inside FP64 refinement
+  const bool h32 = prec != SG_FP64;       // FP32 multigrid hierarchy
+  const size_t T = h32 ? sizeof(float) : sizeof(double);     // hierarchy
+  const size_t KT = mixed ? sizeof(float) : sizeof(double);  // Krylov vectors
   const std::vector<Dims> dims = level_dims(nx, ny);
   I.nx = nx; I.ny = ny; I.nlev = (int)dims.size();
   I.nblk0 = (int)((N + kBlock - 1) / kBlock);
-  I.mixed = mixed ? 1 : 0;
+  I.prec = prec;
   I.G = (double *)take(N * D); I.Lxx = (double *)take(N * D); I.Lxy = (double *)take(N * D);
   I.Lyy = (double *)take(N * D); I.itau = (double *)take(N * D);
   I.f = (double *)take(N * D); I.R = (double *)take(N * D); I.x = (double *)take(N * D);
@@ -74,7 +77,9 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
   I.bcv[0] = (double *)take((ny + 1) * D); I.bcv[1] = (double *)take((ny + 1) * D);
   I.bcv[2] = (double *)take((nx + 1) * D); I.bcv[3] = (double *)take((nx + 1) * D);
   I.partial = (double *)take((size_t)4 * I.nblk0 * D);
-  for (int v = 0; v < kNumVec; ++v) I.kv[v] = (v == VXK && !mixed) ? nullptr : take(N * T);
+  for (int v = 0; v < kNumVec; ++v) I.kv[v] = (v == VXK && !mixed) ? nullptr : take(N * KT);
+  for (int v = 0; v < kNumVecAll - kNumVec; ++v)
+    I.pv[v] = prec == SG_PC32 ? (float *)take(N * sizeof(float)) : nullptr;
   for (int l = 0; l < kMaxLev; ++l) {
     Lev &L = I.lev[l];
     std::memset(&L, 0, sizeof(L));
@@ -84,12 +89,12 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
     if (l == 0) {
       for (int b = 0; b < 5; ++b) {
         L.band64[b] = I.band64[b];
-        L.band[b] = mixed ? take(N * T) : (void *)I.band64[b];
+        L.band[b] = h32 ? take(N * T) : (void *)I.band64[b];
       }
     } else {
       for (int b = 0; b < 9; ++b) {
         L.band64[b] = (double *)take(Nl * D);
-        L.band[b] = mixed ? take(Nl * T) : (void *)L.band64[b];
+        L.band[b] = h32 ? take(Nl * T) : (void *)L.band64[b];
       }
       L.x = take(Nl * T);
       L.b = take(Nl * T);
@@ -109,7 +114,7 @@ static size_t carve(uintptr_t base, int nx, int ny, bool mixed, InstDev &I) {
 struct Slot {
   bool used = false;
   int nx = 0, ny = 0;
-  bool mixed = false;
+  int prec = SG_FP64;
   void *slab = nullptr;
   size_t bytes = 0;
   InstDev dev{};
@@ -195,8 +200,9 @@ struct Batch {
   int rebuild();
   int step(int nsteps, const int64_t *targets, int ntargets);
   int step_once();
-  template <typename T> int krylov(int maxit, bool xstate);
-  template <typename T> void iteration(bool xstate);
+  // KT: Krylov vectors, VT: V-cycle (KT double with VT float is SG_PC32)
+  template <typename KT, typename VT> int krylov(int maxit, bool xstate);
+  template <typename KT, typename VT> void iteration(bool xstate);
   template <typename T> void vcycle(int l, int xid, int bid);
   int read_counter(int which, int *out);
   int scratch(size_t n);
@@ -234,8 +240,8 @@ static void apply_defaults(sg_opts &o) {
 
 int Batch::init(const sg_opts *o, size_t b) {
   if (o) opts = *o;
-  if (opts.precision != SG_FP64 && opts.precision != SG_MIXED)
-    return fail(SG_ERR_ARG, "opts.precision must be SG_FP64 or SG_MIXED");
+  if (opts.precision != SG_FP64 && opts.precision != SG_MIXED && opts.precision != SG_PC32)
+    return fail(SG_ERR_ARG, "opts.precision must be SG_FP64, SG_MIXED or SG_PC32");
   apply_defaults(opts);
   budget = b;
   int ndev = 0;
@@ -369,9 +375,8 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     if (bc->type[s] == SG_DIRICHLET && !bc->values[s])
       return fail(SG_ERR_BC, "Dirichlet BC: missing value.");
   }
-  const bool mixed = opts.precision == SG_MIXED;
   InstDev tmp{};
-  const size_t bytes = carve(0, g->nx, g->ny, mixed, tmp);
+  const size_t bytes = carve(0, g->nx, g->ny, opts.precision, tmp);
   if (budget && in_use + bytes > budget)
     return fail(SG_ERR_NOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
                                   " more bytes, " + std::to_string(budget - in_use) + " free");
@@ -390,8 +395,8 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     return fail(SG_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
   }
   s = Slot{};
-  s.nx = g->nx; s.ny = g->ny; s.mixed = mixed; s.slab = slab; s.bytes = bytes; s.dt = dt;
-  carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, mixed, s.dev);
+  s.nx = g->nx; s.ny = g->ny; s.prec = opts.precision; s.slab = slab; s.bytes = bytes; s.dt = dt;
+  carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, opts.precision, s.dev);
   s.dev.dt = dt;
   s.dev.sc = d_scal + slot;
   s.dev.ticket = d_ticket + 2 * slot;
@@ -647,33 +652,34 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   }
 }
 
-template <typename T> void Batch::iteration(bool xstate) {
+template <typename KT, typename VT> void Batch::iteration(bool xstate) {
   const int nb = dm.ncmap[0], n = (int)order.size();
   const BlkMap *m0 = dm.cmap[0];
-  tbegin(8); launch_bicg<T>(stream, 0, d_insts, m0, nb, xstate); tend(8);
-  vcycle<T>(0, VPH, VP);
-  tbegin(6); launch_bicg<T>(stream, 1, d_insts, m0, nb, xstate); tend(6);
-  launch_finish<T>(stream, FIN_V, d_insts, n, d_cnt);
-  tbegin(8); launch_bicg<T>(stream, 2, d_insts, m0, nb, xstate); tend(8);
-  launch_finish<T>(stream, FIN_S, d_insts, n, d_cnt);
-  vcycle<T>(0, VSH, VS);
-  tbegin(7); launch_bicg<T>(stream, 3, d_insts, m0, nb, xstate); tend(7);
-  launch_finish<T>(stream, FIN_T, d_insts, n, d_cnt);
-  tbegin(8); launch_bicg<T>(stream, 4, d_insts, m0, nb, xstate); tend(8);
-  launch_finish<T>(stream, FIN_X, d_insts, n, d_cnt);
+  constexpr bool pc = sizeof(KT) != sizeof(VT);
+  tbegin(8); launch_bicg<KT>(stream, 0, d_insts, m0, nb, xstate, pc); tend(8);
+  vcycle<VT>(0, pc ? PV_PH : VPH, pc ? PV_B : VP);
+  tbegin(6); launch_bicg<KT>(stream, 1, d_insts, m0, nb, xstate, pc); tend(6);
+  launch_finish<KT>(stream, FIN_V, d_insts, n, d_cnt);
+  tbegin(8); launch_bicg<KT>(stream, 2, d_insts, m0, nb, xstate, pc); tend(8);
+  launch_finish<KT>(stream, FIN_S, d_insts, n, d_cnt);
+  vcycle<VT>(0, pc ? PV_SH : VSH, pc ? PV_B : VS);
+  tbegin(7); launch_bicg<KT>(stream, 3, d_insts, m0, nb, xstate, pc); tend(7);
+  launch_finish<KT>(stream, FIN_T, d_insts, n, d_cnt);
+  tbegin(8); launch_bicg<KT>(stream, 4, d_insts, m0, nb, xstate, pc); tend(8);
+  launch_finish<KT>(stream, FIN_X, d_insts, n, d_cnt);
 }
 
 // Krylov iterations in chunks of check_every. The next chunk is enqueued
 // before the host waits for the active-instance counter of the current one, so
 // the GPU never drains while the host decides; a chunk enqueued after the
 // last instance converged costs only early-exiting launches.
-template <typename T> int Batch::krylov(int maxit, bool xstate) {
+template <typename KT, typename VT> int Batch::krylov(int maxit, bool xstate) {
   if (!ev_cnt[0]) {
     for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&ev_cnt[k], hipEventDisableTiming));
   }
   auto chunk = [&](int slot, int &it) -> int {
     const int c = std::min(opts.check_every, maxit - it);
-    for (int k = 0; k < c; ++k) iteration<T>(xstate);
+    for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
     it += c;
     HIPCHK(hipMemcpyAsync(h_pipe + slot, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cnt[slot], stream));
@@ -709,12 +715,13 @@ int Batch::step_once() {
     part_host = part;
   }
   const bool mixed = opts.precision == SG_MIXED;
+  const bool h32 = opts.precision != SG_FP64;   // FP32 hierarchy and V-cycle
   const double tol2 = mixed ? opts.inner_rtol * opts.inner_rtol : opts.rtol * opts.rtol;
   launch_begin(stream, d_insts, n, d_part, d_cnt, d_cnt + 1, tol2, opts.rtol * opts.rtol,
                mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
   const int nb0 = dm.ncmap[0];
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
-  if (mixed) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
+  if (h32) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
   for (int l = 0; l < dm.maxlev; ++l) {
     tbegin(1);
     // LU factors of the line matrices: smoothing lines of both colours and the
@@ -723,20 +730,22 @@ int Batch::step_once() {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p)
           launch_line_factor(stream, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d,
-                             sh, mixed);
-      launch_line_factor(stream, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, mixed);
-      launch_line_factor(stream, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, mixed);
+                             sh, h32);
+      launch_line_factor(stream, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
+      launch_line_factor(stream, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
     }
     if (l + 1 < dm.maxlev) {
-      launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, mixed);
-      launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, mixed);
+      launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
+      launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
     }
     tend(1);
   }
   if (!mixed) {
     launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0);
     launch_finish<double>(stream, FIN_INIT64, d_insts, n, d_cnt);
-    if ((rc = krylov<double>(opts.max_iters, true))) return rc;
+    if ((rc = h32 ? krylov<double, float>(opts.max_iters, true)
+                  : krylov<double, double>(opts.max_iters, true)))
+      return rc;
   } else {
     for (int ref = 0; ref <= opts.max_iters; ++ref) {
       tbegin(9); launch_ir_residual(stream, d_insts, dm.cmap[0], nb0); tend(9);
@@ -747,7 +756,7 @@ int Batch::step_once() {
       launch_ir_inner_init(stream, d_insts, dm.cmap[0], nb0);
       launch_finish<float>(stream, FIN_INNER_INIT, d_insts, n, d_cnt);
       launch_copy_counter(stream, d_cnt + 1, d_cnt);
-      if ((rc = krylov<float>(opts.max_inner, false))) return rc;
+      if ((rc = krylov<float, float>(opts.max_inner, false))) return rc;
       launch_ir_update(stream, d_insts, dm.cmap[0], nb0);
     }
   }
@@ -887,7 +896,8 @@ int sg_device_count(void) {
 size_t sg_instance_bytes(int nx, int ny, int precision) {
   if (nx < 1 || ny < 1) return 0;
   sg::InstDev tmp{};
-  return sg::carve(0, nx, ny, precision == SG_MIXED, tmp);
+  if (precision != SG_FP64 && precision != SG_MIXED && precision != SG_PC32) return 0;
+  return sg::carve(0, nx, ny, precision, tmp);
 }
 
 int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {

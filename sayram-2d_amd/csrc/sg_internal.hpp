@@ -56,6 +56,10 @@ enum Band { BC = 0, BW = 1, BE = 2, BS = 3, BN = 4, BSW = 5, BSE = 6, BNW = 7, B
 
 // Krylov vector slots (precision of the Krylov iteration).
 enum Vec { VR = 0, VRH, VP, VV, VS, VT, VPH, VSH, VXK, kNumVec };
+// FP32 level-0 vectors of an FP64 Krylov solve with an FP32 V-cycle: the
+// V-cycle input (p or s, scaled) and its outputs for p and s. Ids continue
+// the Krylov slots, so a V-cycle addresses either kind by one index.
+enum PcVec { PV_B = kNumVec, PV_PH, PV_SH, kNumVecAll };
 
 // solver status per instance
 enum Status : int { ST_ACTIVE = 0, ST_HALF = 1, ST_DONE = 2, ST_FAIL = 3, ST_IDLE = 4 };
@@ -64,6 +68,8 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   double rho, rho_old, alpha, omega, beta;
   double bnorm2, rnorm2, tol2;
   double scale;          // mixed: norm of the FP64 residual being corrected
+  double pcs;            // FP32 V-cycle of an FP64 solve: scale of its input
+                         // and output, the current residual norm
   double onorm2, obnorm2, otol2;   // mixed: outer (FP64) residual norms
   double oprev2;         // mixed: residual norm^2 of the previous refinement
   double best2;          // smallest Krylov residual norm^2 of this solve
@@ -90,7 +96,7 @@ struct Lev {            // one multigrid level of one instance
 struct InstDev {
   int nx, ny, nlev, bcmask;   // bcmask bit s set: side s Dirichlet
   int nblk0;                  // level-0 workgroups of this instance
-  int mixed;
+  int prec;                   // sg_precision of the instance
   double dt;
   // geometry (source/Mesh.cc:36-65) and per-vertex interpolation weights of
   // fill_vertex_from_cells (source/Solver.cc:326-381), computed once
@@ -103,6 +109,7 @@ struct InstDev {
   double *f, *R, *x, *r64;
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
+  float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle input and outputs
   double *partial;            // 4 * nblk0
   unsigned int *ticket;       // 2 tickets
   KScal *sc;

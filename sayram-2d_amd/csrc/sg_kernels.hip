@@ -452,10 +452,15 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
 // ---------------------------------------------------------------------------
 // multigrid cycle kernels (precision T)
 // ---------------------------------------------------------------------------
+// level-0 vector by id: a Krylov slot (Vec) or an FP32 V-cycle vector (PcVec)
+__device__ __forceinline__ void *vec(const InstDev &I, int id) {
+  return id < kNumVec ? I.kv[id] : (void *)I.pv[id - kNumVec];
+}
+
 template <typename T>
 __device__ __forceinline__ void level_xb(const InstDev &I, int l, int xid, int bid, T *&x,
                                          const T *&b) {
-  if (l == 0) { x = ptr<T>(I.kv[xid]); b = cptr<T>(I.kv[bid]); }
+  if (l == 0) { x = ptr<T>(vec(I, xid)); b = cptr<T>(vec(I, bid)); }
   else { x = ptr<T>(I.lev[l].x); b = cptr<T>(I.lev[l].b); }
 }
 
@@ -835,14 +840,16 @@ __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j,
   return s;
 }
 
-// M x at slot K with the FP64 level-0 bands
-__device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const double *x) {
+// M x at slot K with the FP64 level-0 bands (x in FP64, or an FP32 vector
+// promoted on load)
+template <typename X>
+__device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const X *x) {
   const int nx = I.nx, hx = half_x(nx);
-  double s = I.band64[BC][K] * x[K];
-  if (i > 0) s += I.band64[BW][K] * x[(size_t)j * nx + xslot(i - 1, hx)];
-  if (i < nx - 1) s += I.band64[BE][K] * x[(size_t)j * nx + xslot(i + 1, hx)];
-  if (j > 0) s += I.band64[BS][K] * x[K - nx];
-  if (j < I.ny - 1) s += I.band64[BN][K] * x[K + nx];
+  double s = I.band64[BC][K] * (double)x[K];
+  if (i > 0) s += I.band64[BW][K] * (double)x[(size_t)j * nx + xslot(i - 1, hx)];
+  if (i < nx - 1) s += I.band64[BE][K] * (double)x[(size_t)j * nx + xslot(i + 1, hx)];
+  if (j > 0) s += I.band64[BS][K] * (double)x[K - nx];
+  if (j < I.ny - 1) s += I.band64[BN][K] * (double)x[K + nx];
   return s;
 }
 
@@ -922,7 +929,7 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
   // weights of parents outside the coarse grid are zero
   const T e = cptr<T>(F.pw[0])[K] * xc[r0 + s0] + cptr<T>(F.pw[1])[K] * xc[r0 + s1] +
               cptr<T>(F.pw[2])[K] * xc[r1 + s0] + cptr<T>(F.pw[3])[K] * xc[r1 + s1];
-  T *x = l == 0 ? ptr<T>(I.kv[xid]) : ptr<T>(F.x);
+  T *x = l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x);
   x[K] = ZERO ? e : x[K] + e;
 }
 
@@ -963,7 +970,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const double r = I.R[K] - spmv5<double>(I, K, i, j, I.x);
+    const double r = I.R[K] - spmv64(I, K, i, j, I.x);   // the FP64 bands in every mode
     ptr<double>(I.kv[VR])[K] = r;
     ptr<double>(I.kv[VRH])[K] = r;
     ptr<double>(I.kv[VP])[K] = 0.0;
@@ -974,8 +981,18 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
   store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
 
-// p = r + beta (p - omega v)
-template <typename T>
+// PC (FP64 Krylov, FP32 V-cycle): the preconditioned vectors ph = M^-1 p and
+// sh = M^-1 s are FP32 V-cycle outputs scaled by pcs, the residual norm of
+// the iteration (the V-cycle input is the vector divided by pcs, so it stays
+// in FP32 range however far the residual has fallen).
+template <typename T, bool PC>
+__device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, size_t K) {
+  if (PC) return (T)(I.sc->pcs * (double)I.pv[pcid - kNumVec][K]);
+  return cptr<T>(I.kv[id])[K];
+}
+
+// p = r + beta (p - omega v); PC: also the scaled FP32 V-cycle input
+template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -985,11 +1002,13 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
   if (K >= I.nx * I.ny) return;
   const T beta = (T)I.sc->beta, omega = (T)I.sc->omega;
   T *p = ptr<T>(I.kv[VP]);
-  p[K] = cptr<T>(I.kv[VR])[K] + beta * (p[K] - omega * cptr<T>(I.kv[VV])[K]);
+  const T pv = cptr<T>(I.kv[VR])[K] + beta * (p[K] - omega * cptr<T>(I.kv[VV])[K]);
+  p[K] = pv;
+  if (PC) I.pv[PV_B - kNumVec][K] = (float)(pv / I.sc->pcs);
 }
 
 // v = M ph; partial (rh, v)
-template <typename T>
+template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -1001,15 +1020,16 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const T v = spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
+    const T v = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_PH - kNumVec])))
+                   : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
     ptr<T>(I.kv[VV])[K] = v;
     acc[0] = cptr<T>(I.kv[VRH])[K] * v;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
 
-// s = r - alpha v; partial (s, s)
-template <typename T>
+// s = r - alpha v; partial (s, s); PC: also the scaled FP32 V-cycle input
+template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -1021,13 +1041,14 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ i
   if (K < N) {
     const T s = cptr<T>(I.kv[VR])[K] - (T)I.sc->alpha * cptr<T>(I.kv[VV])[K];
     ptr<T>(I.kv[VS])[K] = s;
+    if (PC) I.pv[PV_B - kNumVec][K] = (float)(s / I.sc->pcs);
     acc[0] = s * s;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
 
 // t = M sh; partials (t, s), (t, t)
-template <typename T>
+template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -1039,7 +1060,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const T t = spmv5<T>(I, K, i, j, cptr<T>(I.kv[VSH]));
+    const T t = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_SH - kNumVec])))
+                   : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VSH]));
     ptr<T>(I.kv[VT])[K] = t;
     acc[0] = t * cptr<T>(I.kv[VS])[K];
     acc[1] = t * t;
@@ -1050,7 +1072,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
 // x += alpha ph + omega sh; r = s - omega t; partials (rh, r), (r, r).
 // HALF (converged at s): x += alpha ph only. XSTATE: x is the FP64 state,
 // otherwise the inner FP32 correction of mixed precision.
-template <typename T, bool XSTATE>
+template <typename T, bool XSTATE, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
@@ -1063,10 +1085,11 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ i
   T acc[2] = {T(0), T(0)};
   if (K < N) {
     T *x = XSTATE ? (T *)I.x : ptr<T>(I.kv[VXK]);
+    const T ph = precond_vec<T, PC>(I, VPH, PV_PH, K);
     if (st == ST_HALF) {
-      x[K] = x[K] + alpha * cptr<T>(I.kv[VPH])[K];
+      x[K] = x[K] + alpha * ph;
     } else {
-      x[K] = x[K] + alpha * cptr<T>(I.kv[VPH])[K] + omega * cptr<T>(I.kv[VSH])[K];
+      x[K] = x[K] + alpha * ph + omega * precond_vec<T, PC>(I, VSH, PV_SH, K);
       const T r = cptr<T>(I.kv[VS])[K] - omega * cptr<T>(I.kv[VT])[K];
       ptr<T>(I.kv[VR])[K] = r;
       acc[0] = cptr<T>(I.kv[VRH])[K] * r;
@@ -1207,6 +1230,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
       s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
       s.rnorm2 = t[0]; s.bnorm2 = t[1]; s.iters = 0;
       s.best2 = t[0]; s.best_it = 0;
+      s.pcs = t[0] > 0.0 ? sqrt(t[0]) : 1.0;
       if (t[0] <= s.tol2 * t[1]) { s.status = ST_DONE; atomicSub(cnt, 1); }
     }
   } else if (MODE == F_V) {
@@ -1237,6 +1261,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
         s.rho_old = s.rho;
         s.rho = (double)t[0];
         s.rnorm2 = (double)t[1];
+        s.pcs = s.rnorm2 > 0.0 ? sqrt(s.rnorm2) : 1.0;
         s.beta = (double)(((T)s.rho / (T)s.rho_old) * ((T)s.alpha / (T)s.omega));
         // Converged; or stalled at the rounding floor: no halving of the
         // residual for 4 iterations while below 1e-9 relative (or 100x the
@@ -1454,21 +1479,31 @@ template void launch_prolong<float>(hipStream_t, const InstDev *, const BlkMap *
 void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_bicg_init64<<<nb, kBlock, 0, s>>>(d, m);
 }
-template <typename T>
-void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate) {
-  if (!nb) return;
+template <typename T, bool PC>
+static void launch_bicg_t(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb,
+                          bool xstate) {
   switch (which) {
-    case 0: k_bicg_p<T><<<nb, kBlock, 0, s>>>(d, m); break;
-    case 1: k_bicg_v<T><<<nb, kBlock, 0, s>>>(d, m); break;
-    case 2: k_bicg_s<T><<<nb, kBlock, 0, s>>>(d, m); break;
-    case 3: k_bicg_t<T><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 0: k_bicg_p<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 1: k_bicg_v<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 2: k_bicg_s<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 3: k_bicg_t<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
     default:
-      if (xstate) k_bicg_x<T, true><<<nb, kBlock, 0, s>>>(d, m);
-      else k_bicg_x<T, false><<<nb, kBlock, 0, s>>>(d, m);
+      if (xstate) k_bicg_x<T, true, PC><<<nb, kBlock, 0, s>>>(d, m);
+      else k_bicg_x<T, false, PC><<<nb, kBlock, 0, s>>>(d, m);
   }
 }
-template void launch_bicg<double>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool);
-template void launch_bicg<float>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool);
+
+template <typename T>
+void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate,
+                 bool pc) {
+  if (!nb) return;
+  if (pc) launch_bicg_t<double, true>(s, which, d, m, nb, xstate);
+  else launch_bicg_t<T, false>(s, which, d, m, nb, xstate);
+}
+template void launch_bicg<double>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool,
+                                  bool);
+template void launch_bicg<float>(hipStream_t, int, const InstDev *, const BlkMap *, int, bool,
+                                 bool);
 
 template <typename T>
 void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt) {

@@ -27,9 +27,11 @@ template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid);
 void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
-// which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update
+// which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update.
+// pc: FP64 Krylov vectors with FP32 preconditioned vectors (T must be double).
 template <typename T>
-void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate);
+void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate,
+                 bool pc);
 // reduction finish modes (one workgroup per instance)
 enum { FIN_INIT64 = 0, FIN_V, FIN_S, FIN_T, FIN_X, FIN_IR_RES, FIN_INNER_INIT, FIN_FINALIZE };
 template <typename T>

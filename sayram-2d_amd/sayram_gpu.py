@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsayram_gpu.so")
 
 SG_OK, SG_ERR_ARG, SG_ERR_HIP, SG_ERR_NOMEM, SG_ERR_NOCONV, SG_ERR_BC, SG_ERR_STATE = 0, -1, -2, -3, -4, -5, -6
-SG_FP64, SG_MIXED = 0, 1
+SG_FP64, SG_MIXED, SG_PC32 = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 

@@ -60,7 +60,11 @@ def test_instance_bytes_accounting():
     b64 = [sg.Batch.instance_bytes(n, n, sg.SG_FP64) for n in (32, 64, 512, 4096)]
     assert all(x < y for x, y in zip(b64, b64[1:]))
     assert sg.Batch.instance_bytes(4096, 4096, sg.SG_MIXED) < b64[-1]
+    # FP64 Krylov vectors with an FP32 hierarchy sit between the two
+    pc = sg.Batch.instance_bytes(4096, 4096, sg.SG_PC32)
+    assert sg.Batch.instance_bytes(4096, 4096, sg.SG_MIXED) < pc < b64[-1]
     assert sg.Batch.instance_bytes(0, 5, sg.SG_FP64) == 0
+    assert sg.Batch.instance_bytes(8, 8, 3) == 0          # no such precision
     # bounded per-cell footprint (memory sized for many resident instances)
     assert b64[-1] / (4096 * 4096) < 300
 

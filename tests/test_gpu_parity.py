@@ -58,7 +58,7 @@ def test_assembly_bitwise(gpu_lib, shape, kind):
 
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("kind", KINDS)
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_steps_vs_oracle(gpu_lib, shape, kind, precision):
     p = variant(sc.seeded_case(*shape, seed=7, scale=5.0), kind)
     op = orc.OracleProblem(p)
@@ -82,7 +82,7 @@ def test_steps_vs_oracle(gpu_lib, shape, kind, precision):
 def test_batch_equals_solo(gpu_lib):
     probs = [sc.seeded_case(nx, ny, seed=30 + k, scale=s) for k, (nx, ny, s) in
              enumerate([(37, 21, 0.3), (8, 90, 4.0), (64, 64, 1.0), (3, 5, 9.0)])]
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         b = gpu_lib.Batch(gpu_lib.make_opts(precision=prec))
         slots = [b.add(p) for p in probs]
         b.step(3)

@@ -19,7 +19,7 @@ GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(g)[:-4] for g in GOLDEN])
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_gpu_matches_golden(gpu_lib, path, precision):
     p, z = load_golden(path)
     s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=precision))
@@ -32,7 +32,7 @@ def test_gpu_matches_golden(gpu_lib, path, precision):
     s.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_long_trajectory_c1_grid(gpu_lib, precision):
     """The reference's CPU-sized case: 90 (alpha0 5..90 deg) x 100 (ln E,
     0.2..5 MeV), full tensor D, dt = 1e-3 day; 300 steps against the oracle,
@@ -54,20 +54,19 @@ def test_long_trajectory_c1_grid(gpu_lib, precision):
 
 def test_c2_size_properties(gpu_lib):
     """2048 x 2048, tau = 1 day: true residual at the FP64 floor, f >= 0, and
-    the mixed path within 1e-10 of the FP64 path after 10 steps."""
+    the mixed and PC32 paths within 1e-10 of the FP64 path after 10 steps."""
     p = sc.seeded_case(2048, 2048, seed=5, scale=1.0, inv_tau=1.0)
-    a = gpu_lib.Solver(p, gpu_lib.make_opts(precision=0))
-    b = gpu_lib.Solver(p, gpu_lib.make_opts(precision=1))
+    sol = [gpu_lib.Solver(p, gpu_lib.make_opts(precision=prec)) for prec in (0, 1, 2)]
     for _ in range(10):
-        a.update(1)
-        b.update(1)
-        assert a.stats()["last_rel_residual"] <= 1e-12
-        assert b.stats()["last_rel_residual"] <= 1e-12
-    fa, fb = a.f(), b.f()
-    assert fa.min() >= 0.0 and fb.min() >= 0.0
+        for s in sol:
+            s.update(1)
+            assert s.stats()["last_rel_residual"] <= 1e-12
+    fa, fb, fc = (s.f() for s in sol)
+    assert fa.min() >= 0.0 and fb.min() >= 0.0 and fc.min() >= 0.0
     assert rel_l2(fb, fa) <= 1e-10
-    a.close()
-    b.close()
+    assert rel_l2(fc, fa) <= 1e-10
+    for s in sol:
+        s.close()
 
 
 def test_conservation_at_scale(gpu_lib):
@@ -76,7 +75,7 @@ def test_conservation_at_scale(gpu_lib):
     p.bc_type = [sc.ZEROFLUX] * 4
     w = p.G * np.outer(np.diff(p.xe), np.diff(p.ye))
     m0 = np.sum(w * p.f0)
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=prec))
         s.update(5)
         assert abs(np.sum(w * s.f()) - m0) <= 1e-11 * m0
@@ -128,6 +127,9 @@ def test_error_paths(gpu_lib):
     assert rc == sg.SG_ERR_NOCONV and s.stats()["status"] == sg.SG_ERR_NOCONV
     assert s.f().min() >= 0.0
     s.close()
+    with pytest.raises(sg.SayramError) as e:
+        sg.Batch(sg.make_opts(precision=3))
+    assert e.value.code == sg.SG_ERR_ARG
     b = sg.Batch(sg.make_opts(), hbm_budget=sg.Batch.instance_bytes(20, 10) + 1024)
     assert b.add(p) == 0
     assert b.try_add(p) is None          # over the HBM budget: nothing allocated

@@ -17,7 +17,7 @@ def test_poisson_trace_is_seeded_and_bounded():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_continuous_batching_alone_equals_mixed(gpu_lib, precision):
     reqs = sv.poisson_trace(14, 400.0, 16, 96, 2, 9, seed=11)
     # a budget that holds only a few requests at once, so admission waits for
