@@ -3,16 +3,19 @@
 Headline (default) workload: the mixed-precision batch -- 512 independent
 instances with uneven grids (each dimension uniform in 32..512, per-instance D
 seed and amplitude 0.1..10x) plus one 4096 x 4096 grid, advanced together by
-the batched path (FP32 multigrid-preconditioned BiCGSTAB inside FP64
-iterative refinement; FP64 assembly, flux weights, residual and refinement).
-A step is one implicit time step of every instance; inputs are resident in HBM
-before the timed region starts.
+the batched path. The default precision is SG_PC32: FP64 BiCGSTAB (operator,
+vectors, residual, convergence test) preconditioned by an FP32 multigrid
+V-cycle, with FP64 assembly and flux weights. --precision mixed measures the
+FP32 Krylov solve inside FP64 iterative refinement, --precision fp64 the FP64
+solve throughout. A step is one implicit time step of every instance; inputs
+are resident in HBM before the timed region starts.
 
 Beside it, on the same batch:
+  * the batched step in each of the three precisions (a shorter run each);
   * the plain FP64 path -- the same kernels, one instance at a time, FP64
     throughout -- timed for the speed-up;
-  * the error of the mixed result against the plain FP64 GPU result, and of
-    both against the CPU oracle on the smallest instances;
+  * the error of the measured result against the plain FP64 GPU result, and
+    of both against the CPU oracle on the smallest instances;
   * the CPU oracle (banded-LU restatement of Solver::update, one thread) timed
     on a bounded sample of the batch (cpu_baseline);
   * HIP-event timing of the dominant kernel (the level-0 zebra line sweeps)
@@ -39,6 +42,10 @@ import sayram_cases as sc   # noqa: E402
 import sayram_gpu as sg     # noqa: E402
 
 METRIC = "cell-steps/s, batched mixed-prec vs plain FP64 at ≤1e-10 rel err; p99 latency"
+DTYPES = {"pc32": "f64+f32", "mixed": "f32+f64", "fp64": "f64"}
+PREC_NAMES = {"pc32": "FP64 BiCGSTAB with an FP32 multigrid preconditioner",
+              "mixed": "FP32 BiCGSTAB inside FP64 iterative refinement",
+              "fp64": "FP64 throughout"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -70,11 +77,12 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_batched(probs, warmup, steps, precision=sg.SG_MIXED):
-    b = sg.Batch(sg.make_opts(precision=precision))
+def run_batched(probs, warmup, steps, precision=sg.SG_MIXED, graph=True, time_kernel=True):
+    b = sg.Batch(sg.make_opts(precision=precision, use_graph=1 if graph else -1))
     slots = [b.add(p) for p in probs]
     b.step(warmup)
-    b.time_kernel("line_l0")
+    if time_kernel:
+        b.time_kernel("line_l0")
     t0 = time.perf_counter()
     b.step(steps)
     dt = time.perf_counter() - t0
@@ -101,8 +109,8 @@ def run_plain_fp64(probs, warmup, steps):
     return total, fs
 
 
-def mixed_states(probs, nsteps):
-    b = sg.Batch(sg.make_opts(precision=sg.SG_MIXED))
+def batched_states(probs, nsteps, precision):
+    b = sg.Batch(sg.make_opts(precision=precision))
     slots = [b.add(p) for p in probs]
     b.step(nsteps)
     fs = [b.f(s) for s in slots]
@@ -110,8 +118,8 @@ def mixed_states(probs, nsteps):
     return fs
 
 
-def serving(n_req, seed):
-    """Continuous batching under Poisson arrivals (mixed precision), on a
+def serving(n_req, seed, precision, pname):
+    """Continuous batching under Poisson arrivals, on a
     bounded seeded trace. The full serving workload (200 req/s, grids
     log-uniform 32..2048 per side, 10..1000 steps) offers ~2.4e10 cell-steps/s,
     far beyond one GPU, so its latency grows with the trace length; the trace
@@ -120,14 +128,14 @@ def serving(n_req, seed):
     measured serving throughput."""
     import sayram_serve as sv
     reqs = sv.poisson_trace(n_req, 200.0, 32, 512, 10, 100, seed=seed + 17)
-    loop = sv.ServeLoop(hbm_budget=64 << 30, precision=sg.SG_MIXED)
+    loop = sv.ServeLoop(hbm_budget=64 << 30, precision=precision)
     done, wall = loop.run(reqs, realtime=True)
     loop.close()
     s = sv.summarize(done, wall)
     e_cells = ((2048 - 32) / math.log(2048 / 32)) ** 2      # log-uniform sides
     e_steps = (10 + 1000) / 2
     s.update({"trace": f"{n_req} requests, Poisson 200 req/s, sides log-uniform 32..512, "
-                       f"steps 10..100, mixed precision, HBM budget 64 GiB",
+                       f"steps 10..100, precision {pname}, HBM budget 64 GiB",
               "sustainable_rate_full_distribution_req_per_s":
                   s["throughput_cell_steps_per_s"] / (e_cells * e_steps)})
     return s
@@ -167,14 +175,18 @@ def main():
     ap.add_argument("--big", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--precision", choices=["mixed", "fp64", "pc32"], default="mixed",
+    ap.add_argument("--precision", choices=["pc32", "mixed", "fp64"], default="pc32",
                     help="precision of the batched step that is measured")
+    ap.add_argument("--graph", choices=["on", "off"], default="on",
+                    help="replay the Krylov chunks from captured hipGraphs")
+    ap.add_argument("--skip-precisions", action="store_true",
+                    help="do not time the batched step in the other precisions")
     ap.add_argument("--oracle-checks", type=int, default=3)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64,
                     help="requests in the serving trace (0 skips the serving measurement)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="only the timed mixed batched steps (for rocprofv3)")
+                    help="only the timed batched steps (for rocprofv3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times: the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -202,8 +214,12 @@ def main():
     log(f"[bench] {len(probs)} instances, {cells / 1e6:.2f} M cells, inputs in "
         f"{time.perf_counter() - t0:.1f}s")
 
-    prec = {"mixed": sg.SG_MIXED, "fp64": sg.SG_FP64, "pc32": sg.SG_PC32}[args.precision]
-    dt, (kms, kl, kbytes), st = run_batched(probs, args.warmup, args.steps, prec)
+    precs = {"pc32": sg.SG_PC32, "mixed": sg.SG_MIXED, "fp64": sg.SG_FP64}
+    prec = precs[args.precision]
+    graph = args.graph == "on"
+    # HIP-event kernel timing records host-side events, so it runs eagerly
+    dt, (kms, kl, kbytes), st = run_batched(probs, args.warmup, args.steps, prec, graph=graph,
+                                            time_kernel=not graph)
     value = cells * args.steps / dt
     iters = [s["last_iters"] for s in st]
     refs = [s["last_refinements"] for s in st]
@@ -216,11 +232,12 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32+f64",
+           "dtype": DTYPES[args.precision],
            "data": "synthetic: seeded smooth log-normal D tensor, G = sin cos p^2",
-           "config": {"workload": f"batched mixed precision: {args.instances} instances with "
-                                  f"dims uniform in {args.lo}..{args.hi} plus one {args.big}^2 "
-                                  f"grid, dt = 1e-3 day",
+           "config": {"workload": f"batched, {PREC_NAMES[args.precision]}: {args.instances} "
+                                  f"instances with dims uniform in {args.lo}..{args.hi} plus one "
+                                  f"{args.big}^2 grid, dt = 1e-3 day",
+                      "precision": args.precision,
                       "instances": len(probs), "cells": cells, "seed": args.seed},
            "solver": {"inner_iterations_mean": float(np.mean(iters)), "inner_iterations_max": int(max(iters)),
                       "refinements_mean": float(np.mean(refs)),
@@ -236,19 +253,34 @@ def main():
                            "avg_launch_ms": avg_ms, "launches": kl,
                            "algorithmic_bytes_per_launch": kbytes / kl}
 
+    if not args.skip_precisions:
+        out["precisions"] = {}
+        for name, pv in precs.items():
+            if name == args.precision:
+                out["precisions"][name] = {"value": value, "ms_per_step": dt / args.steps * 1e3,
+                                           "krylov_iterations_mean": float(np.mean(iters))}
+                continue
+            d2, _, st2 = run_batched(probs, 1, 3, pv)
+            out["precisions"][name] = {
+                "value": cells * 3 / d2, "ms_per_step": d2 / 3 * 1e3,
+                "krylov_iterations_mean": float(np.mean([s["last_iters"] for s in st2])),
+                "refinements_mean": float(np.mean([s["last_refinements"] for s in st2]))}
+            log(f"[bench] {name} batched: {d2 / 3 * 1e3:.2f} ms/step")
+
     if not args.skip_plain:
         pw, ps = 1, 1
         t_plain, f_plain = run_plain_fp64(probs, pw, ps)
         plain_value = cells * ps / t_plain
         log(f"[bench] plain FP64: {t_plain / ps * 1e3:.1f} ms/step, {plain_value:.3e} cell-steps/s")
-        f_mixed = mixed_states(probs, pw + ps)
-        errs = [rel_l2(a, b) for a, b in zip(f_mixed, f_plain)]
+        f_batched = batched_states(probs, pw + ps, prec)
+        errs = [rel_l2(a, b) for a, b in zip(f_batched, f_plain)]
         out["plain_fp64"] = {"value": plain_value, "unit": "cell-steps/s",
                              "ms_per_step": t_plain / ps * 1e3,
                              "path": "same kernels, one instance at a time, FP64"}
         out["speedup_vs_plain_fp64"] = value / plain_value
-        acc = {"steps_compared": pw + ps, "max_rel_l2_mixed_vs_fp64": float(max(errs)),
-               "min_f_mixed": float(min(f.min() for f in f_mixed))}
+        acc = {"steps_compared": pw + ps, "precision": args.precision,
+               "max_rel_l2_batched_vs_plain_fp64": float(max(errs)),
+               "min_f_batched": float(min(f.min() for f in f_batched))}
         if args.oracle_checks:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle as orc
@@ -256,15 +288,15 @@ def main():
             eo = []
             for k in small:
                 ref = orc.run(orc.OracleProblem(probs[k]), probs[k].f0, pw + ps)
-                eo.append((rel_l2(f_mixed[k], ref), rel_l2(f_plain[k], ref)))
+                eo.append((rel_l2(f_batched[k], ref), rel_l2(f_plain[k], ref)))
             acc["oracle_instances"] = [list(probs[k].f0.shape) for k in small]
-            acc["max_rel_l2_mixed_vs_oracle"] = float(max(e[0] for e in eo))
+            acc["max_rel_l2_batched_vs_oracle"] = float(max(e[0] for e in eo))
             acc["max_rel_l2_fp64_vs_oracle"] = float(max(e[1] for e in eo))
         out["accuracy"] = acc
         log(f"[bench] accuracy: {acc}")
 
     if args.serve_requests > 0:
-        out["serving"] = serving(args.serve_requests, args.seed)
+        out["serving"] = serving(args.serve_requests, args.seed, prec, args.precision)
         log(f"[bench] serving: {out['serving']}")
 
     if args.cpu_budget > 0:

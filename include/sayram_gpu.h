@@ -86,7 +86,9 @@ typedef struct {
   int smoother;         /* bit0: pre x-lines, bit1: pre y-lines, bit2: post x,
                            bit3: post y; default 12 (post-smoothing, x-lines
                            then y-lines) */
-  int use_graph;        /* capture iteration chunks into hipGraphs; default 1 */
+  int use_graph;        /* 0 (default) or 1: each chunk of check_every Krylov
+                           iterations is captured once into a hipGraph and
+                           replayed; -1: launch every kernel eagerly */
   int reserved[6];
 } sg_opts;
 

@@ -172,6 +172,12 @@ struct Batch {
   int *h_cnt = nullptr;         // pinned
   int *h_pipe = nullptr;        // pinned, counters of the pipelined Krylov chunks
   hipEvent_t ev_cnt[2] = {nullptr, nullptr};
+  // Krylov chunks captured as graphs: key (path, xstate, iterations) -> executable.
+  // The graphs hold the map and instance pointers and launch sizes of the
+  // current batch layout, so every rebuild drops them.
+  struct ChunkGraph { int key; hipGraphExec_t exec; };
+  std::vector<ChunkGraph> graphs;
+  void drop_graphs();
   void *d_maps = nullptr;
   size_t d_maps_bytes = 0;
   DevMaps dm{};
@@ -223,6 +229,7 @@ Batch::~Batch() {
   (void)hipFree(d_maps); (void)hipFree(d_scratch);
   if (h_cnt) (void)hipHostFree(h_cnt);
   if (h_pipe) (void)hipHostFree(h_pipe);
+  drop_graphs();
   for (auto ev : ev_cnt)
     if (ev) (void)hipEventDestroy(ev);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
@@ -256,6 +263,7 @@ int Batch::init(const sg_opts *o, size_t b) {
 
 int Batch::ensure_capacity(int cap) {
   if (cap <= capacity) return SG_OK;
+  drop_graphs();
   int nc = std::max(cap, capacity * 2);
   KScal *ns = nullptr;
   unsigned *nt = nullptr;
@@ -450,7 +458,13 @@ int Batch::remove(int slot) {
   return SG_OK;
 }
 
+void Batch::drop_graphs() {
+  for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
+}
+
 int Batch::rebuild() {
+  drop_graphs();
   Maps M;
   order.clear();
   std::vector<InstDev> table;
@@ -679,7 +693,27 @@ template <typename KT, typename VT> int Batch::krylov(int maxit, bool xstate) {
   }
   auto chunk = [&](int slot, int &it) -> int {
     const int c = std::min(opts.check_every, maxit - it);
-    for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
+    if (opts.use_graph >= 0 && time_id < 0) {
+      // replay the captured chunk (kernel timing records host-side events,
+      // so a timed run launches eagerly)
+      const int key = ((sizeof(KT) == 4 ? 0 : sizeof(VT) == 4 ? 1 : 2) * 2 + (xstate ? 1 : 0)) * 4096 + c;
+      hipGraphExec_t exec = nullptr;
+      for (auto &g : graphs)
+        if (g.key == key) exec = g.exec;
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
+        HIPCHK(hipStreamEndCapture(stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+        graphs.push_back({key, exec});
+      }
+      HIPCHK(hipGraphLaunch(exec, stream));
+    } else {
+      for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
+    }
     it += c;
     HIPCHK(hipMemcpyAsync(h_pipe + slot, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cnt[slot], stream));

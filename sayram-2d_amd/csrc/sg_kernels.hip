@@ -33,6 +33,16 @@ template <typename T> __device__ __forceinline__ T &at32(T *base, uint32_t slot)
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + slot * (uint32_t)sizeof(T));
 }
 
+// Workgroup barrier for LDS data only: it waits for this thread's LDS
+// accesses but not for its global loads, so tiles prefetched into registers
+// stay in flight across it. Valid where threads exchange data through LDS
+// only (a thread that reloads a global element is the one that stored it).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // (i, j) of storage slot K of a level with nx columns (hx = half_x(nx))
 __device__ __forceinline__ void cell_ij(int K, int nx, int hx, int &i, int &j) {
   j = K / nx;
@@ -668,13 +678,13 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   // solve layout: thread -> (line sk, steps 4*sc .. 4*sc+3)
   const int sk = tid / W, sc = tid % W;
   const bool sk_ok = sk < cnt;
-  if (tid < L) s_c[tid] = T(0);
-  for (int t0 = 0; t0 < n; t0 += S) {
+  T vb[Q], vlo[Q], viv[Q], vom[Q], vop[Q], xm[Q], xp[Q];
+  T dmm[Q], dmp[Q], dpm[Q], dpp[Q], xmm[Q], xmp[Q], xpm[Q], xpp[Q];
+  // every load of a tile is issued at once, into registers
+  auto load_fwd = [&](int t0) {
     const int tw = min(S, n - t0);
-    T vb[Q], vlo[Q], viv[Q], vom[Q], vop[Q], xm[Q], xp[Q];
-    T dmm[Q], dmp[Q], dpm[Q], dpp[Q], xmm[Q], xmp[Q], xpm[Q], xpp[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {   // issue every load of the tile first
+    for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       vb[q] = vlo[q] = viv[q] = vom[q] = vop[q] = xm[q] = xp[q] = T(0);
       dmm[q] = dmp[q] = dpm[q] = dpp[q] = xmm[q] = xmp[q] = xpm[q] = xpp[q] = T(0);
@@ -702,6 +712,11 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         }
       }
     }
+  };
+  if (tid < L) s_c[tid] = T(0);
+  load_fwd(0);
+  for (int t0 = 0; t0 < n; t0 += S) {
+    const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       T r = vb[q] - vom[q] * xm[q] - vop[q] * xp[q];
@@ -710,7 +725,11 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       s_a[ek[q]][eu[q]] = vlo[q];
       s_i[ek[q]][eu[q]] = viv[q];
     }
-    __syncthreads();
+    // The next tile's loads fly while this one is solved and stored. They
+    // read bands, b and the other colour's lines, which no thread of this
+    // launch writes.
+    if (t0 + S < n) load_fwd(t0 + S);
+    lds_barrier();
     {   // forward: d_t = a_t d_{t-1} + b_t with a_t = -lo_t inv_t, b_t = r_t inv_t
       T al[4], be[4];
       T A = T(1), B = T(0);
@@ -739,24 +758,21 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         d = al[e] * d + be[e];
         if (sk_ok && u < tw) s_r[sk][u] = d;
       }
-      __syncthreads();
+      lds_barrier();
       if (sk_ok && sc == W - 1) s_c[sk] = d;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       if (k < cnt && u < tw)
         at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
     }
-    __syncthreads();
+    lds_barrier();
   }
-  if (tid < L) s_c[tid] = T(0);
-  const int ntiles = (n + S - 1) / S;
-  for (int tt = ntiles - 1; tt >= 0; --tt) {
-    const int t0 = tt * S;
+  T vd[Q], vc[Q];
+  auto load_bwd = [&](int t0) {
     const int tw = min(S, n - t0);
-    T vd[Q], vc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
@@ -767,12 +783,22 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         vc[q] = at32(cu, K);
       }
     }
+  };
+  if (tid < L) s_c[tid] = T(0);
+  const int ntiles = (n + S - 1) / S;
+  // d of every tile was stored above by the thread that loads it here
+  load_bwd((ntiles - 1) * S);
+  for (int tt = ntiles - 1; tt >= 0; --tt) {
+    const int t0 = tt * S;
+    const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       s_r[ek[q]][eu[q]] = vd[q];
       s_a[ek[q]][eu[q]] = vc[q];
     }
-    __syncthreads();
+    // prefetch the tile below (this iteration stores tile tt only)
+    if (tt > 0) load_bwd(t0 - S);
+    lds_barrier();
     {   // backward: x_t = -u_t x_{t+1} + d_t, composed from the line's end
       T al[4], be[4];
       T A = T(1), B = T(0);
@@ -801,17 +827,17 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         xv = al[e] * xv + be[e];
         if (sk_ok && u < tw) s_r[sk][u] = xv;
       }
-      __syncthreads();
+      lds_barrier();
       if (sk_ok && sc == 0) s_c[sk] = xv;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = ek[q], u = eu[q];
       if (k < cnt && u < tw)
         at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
