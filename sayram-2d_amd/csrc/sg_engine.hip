@@ -134,12 +134,31 @@ struct Maps {
 // 64-line x 16-step tiles, longer ones in 16 x 64, long rows in 4 x 256 (rows
 // stay coalesced along the line), long columns in 16 x 256 with 1024 threads
 // (columns keep 16 lines per tile so that the accesses across the lines stay
-// coalesced). It depends only on the instance, so results do not depend on
-// the batch.
-static int tile_shape(int n, int dir) {
+// coalesced). FP64 V-cycles keep the 9-point levels on 256-thread tiles,
+// whose loads fit the registers of 1024-thread tiles only in FP32. It depends
+// only on the instance, its level and the batch's precision, so results do
+// not depend on the rest of the batch.
+static int tile_shape(int n, int dir, int l, bool fp64_vcycle) {
   if (n <= 24) return 0;
   if (dir == 0) return n <= 160 ? 1 : 2;
-  return n <= 384 ? 1 : 3;
+  return (n <= 384 || (fp64_vcycle && l > 0)) ? 1 : 3;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b % 8), each with its own L2. Two neighbouring column groups of one colour
+// share every cache line of their rows (16 columns of 4 bytes fill half a
+// 128-byte line), so in each block of 16 dispatch slots the pair (2x, 2x+1) of
+// the map goes to slots x and x + 8: the same XCD, dispatched together, one
+// fetch per line. The order of the pairs is kept.
+template <typename G> static void xcd_pairs(std::vector<G> &v) {
+  constexpr int kXcd = 8;
+  std::vector<G> out(v);
+  for (size_t s = 0; s + 2 * kXcd <= v.size(); s += 2 * kXcd)
+    for (int x = 0; x < kXcd; ++x) {
+      out[s + x] = v[s + 2 * x];
+      out[s + kXcd + x] = v[s + 2 * x + 1];
+    }
+  v.swap(out);
 }
 
 struct DevMaps {   // offsets into one device buffer
@@ -467,6 +486,7 @@ int Batch::rebuild() {
   drop_graphs();
   Maps M;
   order.clear();
+  const bool f64 = opts.precision == SG_FP64;   // FP64 V-cycle
   std::vector<InstDev> table;
   int maxlev = 0;
   for (size_t k = 0; k < slots.size(); ++k) {
@@ -485,14 +505,14 @@ int Batch::rebuild() {
       }
       if (coarsest) {
         if (ly == 1)
-          M.gcx[l][tile_shape(lx, 0)].push_back({c, 0, 1, 1});
+          M.gcx[l][tile_shape(lx, 0, l, f64)].push_back({c, 0, 1, 1});
         else
-          M.gcy[l][tile_shape(ly, 1)].push_back({c, 0, 1, 1});
+          M.gcy[l][tile_shape(ly, 1, l, f64)].push_back({c, 0, 1, 1});
       } else {
         // groups of tile_lines(shape) lines of one colour
         for (int d = 0; d < 2; ++d) {
           const int nl = d == 0 ? ly : lx;          // number of lines
-          const int sh = tile_shape(d == 0 ? lx : ly, d);
+          const int sh = tile_shape(d == 0 ? lx : ly, d, l, f64);
           const int per = tile_lines(sh);
           for (int p = 0; p < 2; ++p)
             for (int f = p; f < nl; f += 2 * per) {
@@ -516,6 +536,7 @@ int Batch::rebuild() {
           };
           std::stable_sort(v.begin(), v.end(),
                            [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
+          if (d == 1) xcd_pairs(v);
         }
   // one device buffer for all maps
   size_t bytes = 0;
@@ -658,7 +679,9 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
-    launch_prolong<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid);
+    // only the lines the first post-smoothing sweep reads are needed
+    const int rows = post_x ? 1 : post_y ? 2 : 0;
+    launch_prolong<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid, rows);
     tend(5);
     zero = false;
     if (post_x) sweep(0);

@@ -21,16 +21,38 @@ static constexpr double kEps = DBL_EPSILON;   // gEPS, source/common.h:38
 template <typename T> __device__ __forceinline__ T *ptr(void *p) { return static_cast<T *>(p); }
 
 // Element `slot` of an array whose base is uniform across the workgroup, through
-// a 32-bit byte offset: the address is base + zext(offset), the scalar-base form
-// of the global loads and stores, so one offset register serves every array of
-// the same element type. Instances hold at most kMaxCells cells, so the offsets
-// of FP64 arrays stay below 2^31.
-template <typename T> __device__ __forceinline__ const T &at32(const T *base, uint32_t slot) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
-                                      slot * (uint32_t)sizeof(T));
+// a 32-bit byte offset in the global address space. The pointers of an
+// instance are read from its table, so without the cast the compiler must use
+// flat accesses, which also count against the LDS counter: every LDS wait would
+// then wait for the global loads in flight too. Cast, the access is
+// base + zext(offset), the scalar-base form of the global loads and stores, and
+// one offset register serves every array of the same element type. Instances
+// hold at most kMaxCells cells, so the offsets of FP64 arrays stay below 2^31.
+template <typename T> __device__ __forceinline__ T ld32(const T *base, uint32_t slot) {
+  using GC = const __attribute__((address_space(1))) char;
+  using GT = const __attribute__((address_space(1))) T;
+  return *(GT *)((GC *)base + slot * (uint32_t)sizeof(T));
 }
-template <typename T> __device__ __forceinline__ T &at32(T *base, uint32_t slot) {
-  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + slot * (uint32_t)sizeof(T));
+template <typename T> __device__ __forceinline__ void st32(T *base, uint32_t slot, T v) {
+  using GC = __attribute__((address_space(1))) char;
+  using GT = __attribute__((address_space(1))) T;
+  *(GT *)((GC *)base + slot * (uint32_t)sizeof(T)) = v;
+}
+
+// LDS slot of step u of line k in a tile of S steps. A line's steps are stored
+// step-mod-4 major, so when the solve threads, each owning 4 consecutive steps,
+// read their e-th step, consecutive lanes read consecutive words: no bank
+// conflicts (in plain row order the stride of 4 words made every solve access
+// a 4- to 8-way conflict). Loads of 16 lines x 4 steps per wave stay
+// conflict-free too: the row stride S + 4 shifts each line by 4 banks.
+template <int S> __device__ __forceinline__ int tslot(int k, int u) {
+  return k * (S + 4) + (u & 3) * (S / 4 + 1) + (u >> 2);
+}
+
+// Global-address-space view of an array pointer read from the instance table
+// (plain indexing of such a pointer compiles to flat accesses).
+template <typename T> __device__ __forceinline__ __attribute__((address_space(1))) T *gbl(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
 }
 
 // Workgroup barrier for LDS data only: it waits for this thread's LDS
@@ -492,9 +514,9 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
   constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
   constexpr int Q = S * L / NT;      // elements per thread in the loads
   static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
-  __shared__ double s_lo[L][S + 1];
-  __shared__ double s_d[L][S + 1];
-  __shared__ double s_up[L][S + 1];
+  __shared__ double s_lo[L * (S + 4)];
+  __shared__ double s_d[L * (S + 4)];
+  __shared__ double s_up[L * (S + 4)];
   __shared__ double s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
@@ -514,14 +536,13 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
   };
   const int cnt = gr.count;
   const int tid = threadIdx.x;
-  int ek[Q], eu[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int e = q * NT + tid;
-    const int v = e % S;
-    ek[q] = DIR == 0 ? e / S : e % L;
-    eu[q] = DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
-  }
+  // (line, step) of this thread's q-th element (shifts and masks: recomputed
+  // where used rather than held in registers)
+  auto ek = [&](int q) { const int e = q * NT + tid; return DIR == 0 ? e / S : e % L; };
+  auto eu = [&](int q) {
+    const int e = q * NT + tid, v = e % S;
+    return DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
+  };
   const int sk = tid / W, sc = tid % W;
   const bool sk_ok = sk < cnt;
   if (tid < L) s_c[tid] = 0.0;
@@ -530,30 +551,30 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
     double vl[Q], vd[Q], vu[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       vl[q] = 0.0; vd[q] = 1.0; vu[q] = 0.0;
       if (k < cnt && u < tw) {
         const int t = t0 + u;
         const uint32_t K = at(gr.first + k * gr.step, t);
-        vl[q] = t > 0 ? at32(lo, K) : 0.0;
-        vd[q] = at32(bc, K);
-        vu[q] = t < n - 1 ? at32(up, K) : 0.0;
+        vl[q] = t > 0 ? ld32(lo, K) : 0.0;
+        vd[q] = ld32(bc, K);
+        vu[q] = t < n - 1 ? ld32(up, K) : 0.0;
       }
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      s_lo[ek[q]][eu[q]] = vl[q];
-      s_d[ek[q]][eu[q]] = vd[q];
-      s_up[ek[q]][eu[q]] = vu[q];
+      s_lo[tslot<S>(ek(q), eu(q))] = vl[q];
+      s_d[tslot<S>(ek(q), eu(q))] = vd[q];
+      s_up[tslot<S>(ek(q), eu(q))] = vu[q];
     }
-    __syncthreads();
+    lds_barrier();
     // compose this thread's 4 maps: (P, Q) <- [[0, c], [-a, b]] (P, Q)
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int u = 4 * sc + e;
       if (sk_ok && u < tw) {
-        const double a = s_lo[sk][u], bb = s_d[sk][u], c = s_up[sk][u];
+        const double a = s_lo[tslot<S>(sk, u)], bb = s_d[tslot<S>(sk, u)], c = s_up[tslot<S>(sk, u)];
         const double n00 = c * m10, n01 = c * m11;
         const double n10 = -a * m00 + bb * m10, n11 = -a * m01 + bb * m11;
         const double sc_ = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
@@ -585,30 +606,30 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
       const int u = 4 * sc + e;
       ivs[e] = 0.0; cus[e] = 0.0;
       if (sk_ok && u < tw) {
-        const double iv = 1.0 / (s_d[sk][u] - s_lo[sk][u] * cprev);
-        cprev = s_up[sk][u] * iv;
+        const double iv = 1.0 / (s_d[tslot<S>(sk, u)] - s_lo[tslot<S>(sk, u)] * cprev);
+        cprev = s_up[tslot<S>(sk, u)] * iv;
         ivs[e] = iv;
         cus[e] = cprev;
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int u = 4 * sc + e;
-      if (sk_ok && u < tw) { s_d[sk][u] = ivs[e]; s_up[sk][u] = cus[e]; }
+      if (sk_ok && u < tw) { s_d[tslot<S>(sk, u)] = ivs[e]; s_up[tslot<S>(sk, u)] = cus[e]; }
     }
     if (sk_ok && sc == W - 1) s_c[sk] = cprev;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       if (k < cnt && u < tw) {
         const uint32_t K = at(gr.first + k * gr.step, t0 + u);
-        at32(inv, K) = (T)s_d[k][u];
-        at32(cu, K) = (T)s_up[k][u];
+        st32(inv, K, (T)s_d[tslot<S>(k, u)]);
+        st32(cu, K, (T)s_up[tslot<S>(k, u)]);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
@@ -634,9 +655,9 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
   constexpr int Q = S * L / NT;      // elements per thread in the loads
   static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
-  __shared__ T s_r[L][S + 1];
-  __shared__ T s_a[L][S + 1];
-  __shared__ T s_i[L][S + 1];
+  __shared__ T s_r[L * (S + 4)];
+  __shared__ T s_a[L * (S + 4)];
+  __shared__ T s_i[L * (S + 4)];
   __shared__ T s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
@@ -667,14 +688,13 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   const int cnt = gr.count;
   const int tid = threadIdx.x;
   // load/store layout: element e -> (line k, step u), coalesced in memory
-  int ek[Q], eu[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int e = q * NT + tid;
-    const int v = e % S;
-    ek[q] = DIR == 0 ? e / S : e % L;
-    eu[q] = DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
-  }
+  // (line, step) of this thread's q-th element (shifts and masks: recomputed
+  // where used rather than held in registers)
+  auto ek = [&](int q) { const int e = q * NT + tid; return DIR == 0 ? e / S : e % L; };
+  auto eu = [&](int q) {
+    const int e = q * NT + tid, v = e % S;
+    return DIR == 0 ? (v < S / 2 ? 2 * v : 2 * (v - S / 2) + 1) : e / L;
+  };
   // solve layout: thread -> (line sk, steps 4*sc .. 4*sc+3)
   const int sk = tid / W, sc = tid % W;
   const bool sk_ok = sk < cnt;
@@ -685,28 +705,28 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       vb[q] = vlo[q] = viv[q] = vom[q] = vop[q] = xm[q] = xp[q] = T(0);
       dmm[q] = dmp[q] = dpm[q] = dpp[q] = xmm[q] = xmp[q] = xpm[q] = xpp[q] = T(0);
       if (k < cnt && u < tw) {
         const int li = gr.first + k * gr.step, t = t0 + u;
         const uint32_t K = at(li, t);
-        vb[q] = at32(b, K);
-        vlo[q] = t > 0 ? at32(lo, K) : T(0);
-        viv[q] = at32(inv, K);
+        vb[q] = ld32(b, K);
+        vlo[q] = t > 0 ? ld32(lo, K) : T(0);
+        viv[q] = ld32(inv, K);
         if (!ZERO) {
           if (li > 0) {
-            vom[q] = at32(om, K); xm[q] = at32(x, at(li - 1, t));
+            vom[q] = ld32(om, K); xm[q] = ld32(x, at(li - 1, t));
             if (NB == 9) {
-              if (t > 0) { dmm[q] = at32(omm, K); xmm[q] = at32(x, at(li - 1, t - 1)); }
-              if (t < n - 1) { dmp[q] = at32(omp, K); xmp[q] = at32(x, at(li - 1, t + 1)); }
+              if (t > 0) { dmm[q] = ld32(omm, K); xmm[q] = ld32(x, at(li - 1, t - 1)); }
+              if (t < n - 1) { dmp[q] = ld32(omp, K); xmp[q] = ld32(x, at(li - 1, t + 1)); }
             }
           }
           if (li < nperp - 1) {
-            vop[q] = at32(op, K); xp[q] = at32(x, at(li + 1, t));
+            vop[q] = ld32(op, K); xp[q] = ld32(x, at(li + 1, t));
             if (NB == 9) {
-              if (t > 0) { dpm[q] = at32(opm, K); xpm[q] = at32(x, at(li + 1, t - 1)); }
-              if (t < n - 1) { dpp[q] = at32(opp, K); xpp[q] = at32(x, at(li + 1, t + 1)); }
+              if (t > 0) { dpm[q] = ld32(opm, K); xpm[q] = ld32(x, at(li + 1, t - 1)); }
+              if (t < n - 1) { dpp[q] = ld32(opp, K); xpp[q] = ld32(x, at(li + 1, t + 1)); }
             }
           }
         }
@@ -721,9 +741,9 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     for (int q = 0; q < Q; ++q) {
       T r = vb[q] - vom[q] * xm[q] - vop[q] * xp[q];
       if (NB == 9) r = r - dmm[q] * xmm[q] - dmp[q] * xmp[q] - dpm[q] * xpm[q] - dpp[q] * xpp[q];
-      s_r[ek[q]][eu[q]] = r;
-      s_a[ek[q]][eu[q]] = vlo[q];
-      s_i[ek[q]][eu[q]] = viv[q];
+      s_r[tslot<S>(ek(q), eu(q))] = r;
+      s_a[tslot<S>(ek(q), eu(q))] = vlo[q];
+      s_i[tslot<S>(ek(q), eu(q))] = viv[q];
     }
     // The next tile's loads fly while this one is solved and stored. They
     // read bands, b and the other colour's lines, which no thread of this
@@ -738,8 +758,8 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         const int u = 4 * sc + e;
         al[e] = T(1); be[e] = T(0);
         if (sk_ok && u < tw) {
-          al[e] = -s_a[sk][u] * s_i[sk][u];
-          be[e] = s_r[sk][u] * s_i[sk][u];
+          al[e] = -s_a[tslot<S>(sk, u)] * s_i[tslot<S>(sk, u)];
+          be[e] = s_r[tslot<S>(sk, u)] * s_i[tslot<S>(sk, u)];
         }
         B = al[e] * B + be[e];
         A = al[e] * A;
@@ -756,7 +776,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       for (int e = 0; e < 4; ++e) {
         const int u = 4 * sc + e;
         d = al[e] * d + be[e];
-        if (sk_ok && u < tw) s_r[sk][u] = d;
+        if (sk_ok && u < tw) s_r[tslot<S>(sk, u)] = d;
       }
       lds_barrier();
       if (sk_ok && sc == W - 1) s_c[sk] = d;
@@ -764,9 +784,9 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     lds_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       if (k < cnt && u < tw)
-        at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
+        st32(x, at(gr.first + k * gr.step, t0 + u), s_r[tslot<S>(k, u)]);
     }
     lds_barrier();
   }
@@ -775,12 +795,12 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       vd[q] = vc[q] = T(0);
       if (k < cnt && u < tw) {
         const uint32_t K = at(gr.first + k * gr.step, t0 + u);
-        vd[q] = at32(x, K);
-        vc[q] = at32(cu, K);
+        vd[q] = ld32(x, K);
+        vc[q] = ld32(cu, K);
       }
     }
   };
@@ -793,8 +813,8 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      s_r[ek[q]][eu[q]] = vd[q];
-      s_a[ek[q]][eu[q]] = vc[q];
+      s_r[tslot<S>(ek(q), eu(q))] = vd[q];
+      s_a[tslot<S>(ek(q), eu(q))] = vc[q];
     }
     // prefetch the tile below (this iteration stores tile tt only)
     if (tt > 0) load_bwd(t0 - S);
@@ -807,8 +827,8 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
         const int u = 4 * sc + e;
         al[e] = T(1); be[e] = T(0);
         if (sk_ok && u < tw) {
-          al[e] = -s_a[sk][u];
-          be[e] = s_r[sk][u];
+          al[e] = -s_a[tslot<S>(sk, u)];
+          be[e] = s_r[tslot<S>(sk, u)];
         }
         B = al[e] * B + be[e];
         A = al[e] * A;
@@ -825,7 +845,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       for (int e = 3; e >= 0; --e) {
         const int u = 4 * sc + e;
         xv = al[e] * xv + be[e];
-        if (sk_ok && u < tw) s_r[sk][u] = xv;
+        if (sk_ok && u < tw) s_r[tslot<S>(sk, u)] = xv;
       }
       lds_barrier();
       if (sk_ok && sc == 0) s_c[sk] = xv;
@@ -833,9 +853,9 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     lds_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int k = ek[q], u = eu[q];
+      const int k = ek(q), u = eu(q);
       if (k < cnt && u < tw)
-        at32(x, at(gr.first + k * gr.step, t0 + u)) = s_r[k][u];
+        st32(x, at(gr.first + k * gr.step, t0 + u), s_r[tslot<S>(k, u)]);
     }
     lds_barrier();
   }
@@ -843,24 +863,25 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
 
 // (A x)(i, j) at slot K; kw, ke are the slots of (i-1, j) and (i+1, j)
 template <typename T, int NB>
-__device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j, const T *x) {
+__device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j, const T *xp) {
   const int nx = L.nx, ny = L.ny, hx = half_x(nx);
   const size_t kw = (size_t)j * nx + xslot(i - 1, hx), ke = (size_t)j * nx + xslot(i + 1, hx);
-  T s = cptr<T>(L.band[BC])[K] * x[K];
-  if (i > 0) s += cptr<T>(L.band[BW])[K] * x[kw];
-  if (i < nx - 1) s += cptr<T>(L.band[BE])[K] * x[ke];
+  const auto x = gbl(xp);
+  T s = gbl(cptr<T>(L.band[BC]))[K] * x[K];
+  if (i > 0) s += gbl(cptr<T>(L.band[BW]))[K] * x[kw];
+  if (i < nx - 1) s += gbl(cptr<T>(L.band[BE]))[K] * x[ke];
   if (j > 0) {
-    s += cptr<T>(L.band[BS])[K] * x[K - nx];
+    s += gbl(cptr<T>(L.band[BS]))[K] * x[K - nx];
     if (NB == 9) {
-      if (i > 0) s += cptr<T>(L.band[BSW])[K] * x[kw - nx];
-      if (i < nx - 1) s += cptr<T>(L.band[BSE])[K] * x[ke - nx];
+      if (i > 0) s += gbl(cptr<T>(L.band[BSW]))[K] * x[kw - nx];
+      if (i < nx - 1) s += gbl(cptr<T>(L.band[BSE]))[K] * x[ke - nx];
     }
   }
   if (j < ny - 1) {
-    s += cptr<T>(L.band[BN])[K] * x[K + nx];
+    s += gbl(cptr<T>(L.band[BN]))[K] * x[K + nx];
     if (NB == 9) {
-      if (i > 0) s += cptr<T>(L.band[BNW])[K] * x[kw + nx];
-      if (i < nx - 1) s += cptr<T>(L.band[BNE])[K] * x[ke + nx];
+      if (i > 0) s += gbl(cptr<T>(L.band[BNW]))[K] * x[kw + nx];
+      if (i < nx - 1) s += gbl(cptr<T>(L.band[BNE]))[K] * x[ke + nx];
     }
   }
   return s;
@@ -869,13 +890,14 @@ __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j,
 // M x at slot K with the FP64 level-0 bands (x in FP64, or an FP32 vector
 // promoted on load)
 template <typename X>
-__device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const X *x) {
+__device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const X *xp) {
   const int nx = I.nx, hx = half_x(nx);
-  double s = I.band64[BC][K] * (double)x[K];
-  if (i > 0) s += I.band64[BW][K] * (double)x[(size_t)j * nx + xslot(i - 1, hx)];
-  if (i < nx - 1) s += I.band64[BE][K] * (double)x[(size_t)j * nx + xslot(i + 1, hx)];
-  if (j > 0) s += I.band64[BS][K] * (double)x[K - nx];
-  if (j < I.ny - 1) s += I.band64[BN][K] * (double)x[K + nx];
+  const auto x = gbl(xp);
+  double s = gbl(I.band64[BC])[K] * (double)x[K];
+  if (i > 0) s += gbl(I.band64[BW])[K] * (double)x[(size_t)j * nx + xslot(i - 1, hx)];
+  if (i < nx - 1) s += gbl(I.band64[BE])[K] * (double)x[(size_t)j * nx + xslot(i + 1, hx)];
+  if (j > 0) s += gbl(I.band64[BS])[K] * (double)x[K - nx];
+  if (j < I.ny - 1) s += gbl(I.band64[BN])[K] * (double)x[K + nx];
   return s;
 }
 
@@ -901,6 +923,11 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
 
 // b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J).
 // FROM_B: x_l is still zero, so r_l is the level's right-hand side itself.
+// Work-items take the coarse points of a row in column order (not storage
+// order): the fine columns 2I and 2I +- 1 of consecutive work-items are then
+// consecutive in the fine row's even and odd halves, so every load is
+// contiguous across the wave; the coarse store alternates between the two
+// contiguous halves of the coarse row.
 template <typename T, bool FROM_B>
 __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
@@ -912,28 +939,40 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   const Lev &Cl = I.lev[l + 1];
   const int KC = m.k0 + threadIdx.x;
   if (KC >= Cl.nx * Cl.ny) return;
-  int Ic, J;
-  cell_ij(KC, Cl.nx, half_x(Cl.nx), Ic, J);
+  const int J = KC / Cl.nx, Ic = KC - J * Cl.nx;    // column order
   const int hxf = half_x(F.nx);
   T *x; const T *b;
   level_xb<T>(I, l, xid, bid, x, b);
-  const T *__restrict__ r = FROM_B ? b : cptr<T>(F.cp);
+  const auto r = gbl(FROM_B ? b : cptr<T>(F.cp));
+  const auto pw0 = gbl(cptr<T>(F.pw[0])), pw1 = gbl(cptr<T>(F.pw[1]));
+  const auto pw2 = gbl(cptr<T>(F.pw[2])), pw3 = gbl(cptr<T>(F.pw[3]));
   T s = T(0);
-  for (int fj = 2 * J - 1; fj <= 2 * J + 1; ++fj) {
+#pragma unroll
+  for (int db = -1; db <= 1; ++db) {
+    const int fj = 2 * J + db;
     if (fj < 0 || fj >= F.ny) continue;
-    const int dJ = J - (fj >> 1);
-    for (int fi = 2 * Ic - 1; fi <= 2 * Ic + 1; ++fi) {
+#pragma unroll
+    for (int da = -1; da <= 1; ++da) {
+      const int fi = 2 * Ic + da;
       if (fi < 0 || fi >= F.nx) continue;
-      const int slot = (Ic - (fi >> 1)) + 2 * dJ;
+      // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
+      const int slot = (da < 0 ? 1 : 0) + 2 * (db < 0 ? 1 : 0);
       const size_t kf = (size_t)fj * F.nx + xslot(fi, hxf);
-      s += cptr<T>(F.pw[slot])[kf] * r[kf];
+      const auto w = slot == 0 ? pw0 : slot == 1 ? pw1 : slot == 2 ? pw2 : pw3;
+      s += w[kf] * r[kf];
     }
   }
-  ptr<T>(Cl.b)[KC] = s;
+  gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
 }
 
 // x_l += P x_{l+1}. ZERO: x_l = P x_{l+1}.
-template <typename T, bool ZERO>
+// ROWS 0: every cell. ROWS 1: odd rows only; ROWS 2: odd columns only. A
+// zebra line sweep solves its colour-0 lines from their neighbours and never
+// reads the lines it solves, and the colour-1 lines it reads are solved next:
+// after the prolongation only the colour-1 lines of the first post-smoothing
+// sweep are read (rows 1, 3, .. for x-lines; columns 1, 3, .. for y-lines).
+// The block map covers the level; blocks past the compact range exit.
+template <typename T, bool ZERO, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ insts,
                                                    const BlkMap *__restrict__ map, int l,
                                                    int xid) {
@@ -942,20 +981,34 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
   if (I.sc->status != ST_ACTIVE) return;
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
-  const int K = m.k0 + threadIdx.x;
-  if (K >= F.nx * F.ny) return;
-  int i, j;
-  cell_ij(K, F.nx, half_x(F.nx), i, j);
+  const int idx = m.k0 + threadIdx.x;
+  const int hx = half_x(F.nx);
+  int K, i, j;
+  if (ROWS == 1) {            // rows 1, 3, ..: nx cells each
+    if (idx >= F.nx * (F.ny >> 1)) return;
+    const int r = idx / F.nx, p = idx - r * F.nx;
+    j = 2 * r + 1; i = xcol(p, hx); K = j * F.nx + p;
+  } else if (ROWS == 2) {     // columns 1, 3, ..: the second half of each row
+    const int no = F.nx - hx;
+    if (idx >= no * F.ny) return;
+    j = idx / no;
+    const int p = hx + (idx - j * no);
+    i = xcol(p, hx); K = j * F.nx + p;
+  } else {
+    K = idx;
+    if (K >= F.nx * F.ny) return;
+    cell_ij(K, F.nx, hx, i, j);
+  }
   const int I0 = i >> 1, J0 = j >> 1;
   const int I1 = I0 + 1 < Cl.nx ? I0 + 1 : I0, J1 = J0 + 1 < Cl.ny ? J0 + 1 : J0;
-  const T *__restrict__ xc = cptr<T>(Cl.x);
+  const auto xc = gbl(cptr<T>(Cl.x));
   const int hxc = half_x(Cl.nx);
   const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
   const int s0 = xslot(I0, hxc), s1 = xslot(I1, hxc);
   // weights of parents outside the coarse grid are zero
-  const T e = cptr<T>(F.pw[0])[K] * xc[r0 + s0] + cptr<T>(F.pw[1])[K] * xc[r0 + s1] +
-              cptr<T>(F.pw[2])[K] * xc[r1 + s0] + cptr<T>(F.pw[3])[K] * xc[r1 + s1];
-  T *x = l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x);
+  const T e = gbl(cptr<T>(F.pw[0]))[K] * xc[r0 + s0] + gbl(cptr<T>(F.pw[1]))[K] * xc[r0 + s1] +
+              gbl(cptr<T>(F.pw[2]))[K] * xc[r1 + s0] + gbl(cptr<T>(F.pw[3]))[K] * xc[r1 + s1];
+  const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x));
   x[K] = ZERO ? e : x[K] + e;
 }
 
@@ -996,13 +1049,13 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const double r = I.R[K] - spmv64(I, K, i, j, I.x);   // the FP64 bands in every mode
-    ptr<double>(I.kv[VR])[K] = r;
-    ptr<double>(I.kv[VRH])[K] = r;
-    ptr<double>(I.kv[VP])[K] = 0.0;
-    ptr<double>(I.kv[VV])[K] = 0.0;
+    const double r = gbl(I.R)[K] - spmv64(I, K, i, j, I.x);   // the FP64 bands in every mode
+    gbl(ptr<double>(I.kv[VR]))[K] = r;
+    gbl(ptr<double>(I.kv[VRH]))[K] = r;
+    gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
+    gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
     v[0] = r * r;
-    v[1] = I.R[K] * I.R[K];
+    v[1] = gbl(I.R)[K] * gbl(I.R)[K];
   }
   store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
@@ -1013,8 +1066,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
 // in FP32 range however far the residual has fallen).
 template <typename T, bool PC>
 __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, size_t K) {
-  if (PC) return (T)(I.sc->pcs * (double)I.pv[pcid - kNumVec][K]);
-  return cptr<T>(I.kv[id])[K];
+  if (PC) return (T)(I.sc->pcs * (double)gbl(I.pv[pcid - kNumVec])[K]);
+  return gbl(cptr<T>(I.kv[id]))[K];
 }
 
 // p = r + beta (p - omega v); PC: also the scaled FP32 V-cycle input
@@ -1028,9 +1081,9 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
   if (K >= I.nx * I.ny) return;
   const T beta = (T)I.sc->beta, omega = (T)I.sc->omega;
   T *p = ptr<T>(I.kv[VP]);
-  const T pv = cptr<T>(I.kv[VR])[K] + beta * (p[K] - omega * cptr<T>(I.kv[VV])[K]);
-  p[K] = pv;
-  if (PC) I.pv[PV_B - kNumVec][K] = (float)(pv / I.sc->pcs);
+  const T pv = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
+  gbl(p)[K] = pv;
+  if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(pv / I.sc->pcs);
 }
 
 // v = M ph; partial (rh, v)
@@ -1048,8 +1101,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
     cell_ij(K, I.nx, half_x(I.nx), i, j);
     const T v = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_PH - kNumVec])))
                    : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
-    ptr<T>(I.kv[VV])[K] = v;
-    acc[0] = cptr<T>(I.kv[VRH])[K] * v;
+    gbl(ptr<T>(I.kv[VV]))[K] = v;
+    acc[0] = gbl(cptr<T>(I.kv[VRH]))[K] * v;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
@@ -1065,9 +1118,9 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ i
   const int N = I.nx * I.ny;
   T acc[1] = {T(0)};
   if (K < N) {
-    const T s = cptr<T>(I.kv[VR])[K] - (T)I.sc->alpha * cptr<T>(I.kv[VV])[K];
-    ptr<T>(I.kv[VS])[K] = s;
-    if (PC) I.pv[PV_B - kNumVec][K] = (float)(s / I.sc->pcs);
+    const T s = gbl(cptr<T>(I.kv[VR]))[K] - (T)I.sc->alpha * gbl(cptr<T>(I.kv[VV]))[K];
+    gbl(ptr<T>(I.kv[VS]))[K] = s;
+    if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(s / I.sc->pcs);
     acc[0] = s * s;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
@@ -1088,8 +1141,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
     cell_ij(K, I.nx, half_x(I.nx), i, j);
     const T t = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_SH - kNumVec])))
                    : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VSH]));
-    ptr<T>(I.kv[VT])[K] = t;
-    acc[0] = t * cptr<T>(I.kv[VS])[K];
+    gbl(ptr<T>(I.kv[VT]))[K] = t;
+    acc[0] = t * gbl(cptr<T>(I.kv[VS]))[K];
     acc[1] = t * t;
   }
   store_partials<T, 2>(I, m.k0 / kBlock, acc);
@@ -1113,12 +1166,12 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ i
     T *x = XSTATE ? (T *)I.x : ptr<T>(I.kv[VXK]);
     const T ph = precond_vec<T, PC>(I, VPH, PV_PH, K);
     if (st == ST_HALF) {
-      x[K] = x[K] + alpha * ph;
+      gbl(x)[K] = gbl(x)[K] + alpha * ph;
     } else {
-      x[K] = x[K] + alpha * ph + omega * precond_vec<T, PC>(I, VSH, PV_SH, K);
-      const T r = cptr<T>(I.kv[VS])[K] - omega * cptr<T>(I.kv[VT])[K];
-      ptr<T>(I.kv[VR])[K] = r;
-      acc[0] = cptr<T>(I.kv[VRH])[K] * r;
+      gbl(x)[K] = gbl(x)[K] + alpha * ph + omega * precond_vec<T, PC>(I, VSH, PV_SH, K);
+      const T r = gbl(cptr<T>(I.kv[VS]))[K] - omega * gbl(cptr<T>(I.kv[VT]))[K];
+      gbl(ptr<T>(I.kv[VR]))[K] = r;
+      acc[0] = gbl(cptr<T>(I.kv[VRH]))[K] * r;
       acc[1] = r * r;
     }
   }
@@ -1137,10 +1190,10 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restric
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const double r = I.R[K] - spmv64(I, K, i, j, I.x);
-    I.r64[K] = r;
+    const double r = gbl(I.R)[K] - spmv64(I, K, i, j, I.x);
+    gbl(I.r64)[K] = r;
     v[0] = r * r;
-    v[1] = I.R[K] * I.R[K];
+    v[1] = gbl(I.R)[K] * gbl(I.R)[K];
   }
   store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
@@ -1155,12 +1208,12 @@ __global__ void __launch_bounds__(kBlock) k_ir_inner_init(const InstDev *__restr
   const int N = I.nx * I.ny;
   float v[1] = {0.0f};
   if (K < N) {
-    const float r = (float)(I.r64[K] / I.sc->scale);
-    ptr<float>(I.kv[VR])[K] = r;
-    ptr<float>(I.kv[VRH])[K] = r;
-    ptr<float>(I.kv[VP])[K] = 0.0f;
-    ptr<float>(I.kv[VV])[K] = 0.0f;
-    ptr<float>(I.kv[VXK])[K] = 0.0f;
+    const float r = (float)(gbl(I.r64)[K] / I.sc->scale);
+    gbl(ptr<float>(I.kv[VR]))[K] = r;
+    gbl(ptr<float>(I.kv[VRH]))[K] = r;
+    gbl(ptr<float>(I.kv[VP]))[K] = 0.0f;
+    gbl(ptr<float>(I.kv[VV]))[K] = 0.0f;
+    gbl(ptr<float>(I.kv[VXK]))[K] = 0.0f;
     v[0] = r * r;
   }
   store_partials<float, 1>(I, m.k0 / kBlock, v);
@@ -1174,7 +1227,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_update(const InstDev *__restrict_
   if (I.sc->ostatus != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   if (K >= I.nx * I.ny) return;
-  I.x[K] += I.sc->scale * (double)ptr<float>(I.kv[VXK])[K];
+  gbl(I.x)[K] += I.sc->scale * (double)gbl(ptr<float>(I.kv[VXK]))[K];
 }
 
 // f = max(x, 0); partials of the true residual of the unclamped x, ||R||, the
@@ -1191,10 +1244,10 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const double xv = I.x[K];
-    const double r = I.R[K] - spmv64(I, K, i, j, I.x);
+    const double xv = gbl(I.x)[K];
+    const double r = gbl(I.R)[K] - spmv64(I, K, i, j, I.x);
     v[0] = r * r;
-    v[1] = I.R[K] * I.R[K];
+    v[1] = gbl(I.R)[K] * gbl(I.R)[K];
     v[2] = xv < 0.0 ? 1.0 : 0.0;
     mn = xv;
   }
@@ -1209,8 +1262,8 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
     I.partial[(size_t)3 * I.nblk0 + m.k0 / kBlock] = b;
   }
   if (K < N) {
-    const double xv = I.x[K];
-    I.f[K] = xv > 0.0 ? xv : 0.0;   // min f >= 0 enforced exactly
+    const double xv = gbl(I.x)[K];
+    gbl(I.f)[K] = xv > 0.0 ? xv : 0.0;   // min f >= 0 enforced exactly
   }
 }
 
@@ -1492,15 +1545,17 @@ template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap 
 
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
-                    int xid) {
+                    int xid, int rows) {
   if (!nb) return;
-  if (zero) k_prolong<T, true><<<nb, kBlock, 0, s>>>(d, m, l, xid);
-  else k_prolong<T, false><<<nb, kBlock, 0, s>>>(d, m, l, xid);
+#define SG_PRO(Z, R) k_prolong<T, Z, R><<<nb, kBlock, 0, s>>>(d, m, l, xid)
+  if (zero) { if (rows == 1) SG_PRO(true, 1); else if (rows == 2) SG_PRO(true, 2); else SG_PRO(true, 0); }
+  else { if (rows == 1) SG_PRO(false, 1); else if (rows == 2) SG_PRO(false, 2); else SG_PRO(false, 0); }
+#undef SG_PRO
 }
 template void launch_prolong<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                     int);
+                                     int, int);
 template void launch_prolong<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                    int);
+                                    int, int);
 
 void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_bicg_init64<<<nb, kBlock, 0, s>>>(d, m);

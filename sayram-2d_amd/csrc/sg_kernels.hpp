@@ -23,9 +23,11 @@ void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, i
 template <typename T>
 void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
                      int xid, int bid);
+// rows: 0 every cell, 1 odd rows only, 2 odd columns only (the cells the first
+// post-smoothing sweep reads)
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
-                    int xid);
+                    int xid, int rows);
 void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 // which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update.
 // pc: FP64 Krylov vectors with FP32 preconditioned vectors (T must be double).
