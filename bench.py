@@ -77,8 +77,10 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_batched(probs, warmup, steps, precision=sg.SG_MIXED, graph=True, time_kernel=True):
-    b = sg.Batch(sg.make_opts(precision=precision, use_graph=1 if graph else -1))
+def run_batched(probs, warmup, steps, precision=sg.SG_MIXED, graph=True, time_kernel=True,
+                opts=None):
+    b = sg.Batch(opts if opts is not None else
+                 sg.make_opts(precision=precision, use_graph=1 if graph else -1))
     slots = [b.add(p) for p in probs]
     b.step(warmup)
     if time_kernel:
@@ -179,6 +181,9 @@ def main():
                     help="precision of the batched step that is measured")
     ap.add_argument("--graph", choices=["on", "off"], default="on",
                     help="replay the Krylov chunks from captured hipGraphs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch parameters on the workload first (sayram_tune) and "
+                         "time the step with the best set")
     ap.add_argument("--skip-precisions", action="store_true",
                     help="do not time the batched step in the other precisions")
     ap.add_argument("--oracle-checks", type=int, default=3)
@@ -217,9 +222,19 @@ def main():
     precs = {"pc32": sg.SG_PC32, "mixed": sg.SG_MIXED, "fp64": sg.SG_FP64}
     prec = precs[args.precision]
     graph = args.graph == "on"
+    tuned = None
+    if args.autotune:
+        import sayram_tune as tu
+        best, bv, trials = tu.tune(probs, prec, steps=1, warmup=1, log=log)
+        tuned = {"best": best, "value": bv, "trials": len(trials)}
+        graph = best["use_graph"] == 1
+    opts = None
+    if tuned:
+        p = dict(tuned["best"], use_graph=1 if graph else -1)
+        opts = tu.opts_for(prec, p)
     # HIP-event kernel timing records host-side events, so it runs eagerly
     dt, (kms, kl, kbytes), st = run_batched(probs, args.warmup, args.steps, prec, graph=graph,
-                                            time_kernel=not graph)
+                                            time_kernel=not graph, opts=opts)
     value = cells * args.steps / dt
     iters = [s["last_iters"] for s in st]
     refs = [s["last_refinements"] for s in st]
@@ -239,15 +254,26 @@ def main():
                                   f"{args.big}^2 grid, dt = 1e-3 day",
                       "precision": args.precision,
                       "instances": len(probs), "cells": cells, "seed": args.seed},
+           "launch": {"graph": graph, "autotuned": tuned},
            "solver": {"inner_iterations_mean": float(np.mean(iters)), "inner_iterations_max": int(max(iters)),
                       "refinements_mean": float(np.mean(refs)),
                       "min_f": float(min(s["min_f"] for s in st)),
                       "clamped_cells": int(sum(s["clamp_count"] for s in st)),
                       "max_rel_residual": float(max(s["last_rel_residual"] for s in st))}}
+    if graph:
+        # HIP-event timing of the dominant kernel is recorded from the host, so
+        # it runs over an equal timed region of eager launches (same kernels,
+        # same launch sizes); its step time is reported beside the graph run's
+        eopts = None if opts is None else tu.opts_for(prec, dict(tuned["best"], use_graph=-1))
+        dte, (kms, kl, kbytes), _ = run_batched(probs, args.warmup, args.steps, prec, graph=False,
+                                                time_kernel=True, opts=eopts)
+        out["launch"]["eager_ms_per_step"] = dte / args.steps * 1e3
+        log(f"[bench] eager launches (kernel timing): {dte / args.steps * 1e3:.2f} ms/step")
     if kl and kms > 0:
         avg_ms = kms / kl
         achieved = kbytes / (kms * 1e-3) / 1e9
-        out["roofline"] = {"kernel": "k_line<float,5,*> level-0 zebra line sweep (all instances)",
+        out["roofline"] = {"kernel": "k_line<float,5,*,false,*> level-0 zebra line sweeps (all "
+                                     "instances, every tile shape)",
                            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_launch_ms": avg_ms, "launches": kl,

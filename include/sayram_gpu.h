@@ -82,14 +82,21 @@ typedef struct {
                            refinement sweeps (MIXED); default 200 / 12 */
   double inner_rtol;    /* MIXED: FP32 inner solve tolerance; default 1e-5 */
   int max_inner;        /* MIXED: FP32 inner iteration cap; default 40 */
-  int check_every;      /* iterations between host convergence checks; default 2 */
+  int check_every;      /* iterations between host convergence checks; default 1
+                           (the checks are pipelined one chunk ahead) */
   int smoother;         /* bit0: pre x-lines, bit1: pre y-lines, bit2: post x,
                            bit3: post y; default 12 (post-smoothing, x-lines
                            then y-lines) */
   int use_graph;        /* 0 (default) or 1: each chunk of check_every Krylov
                            iterations is captured once into a hipGraph and
                            replayed; -1: launch every kernel eagerly */
-  int reserved[6];
+  int line_tiles[4];    /* line-solve tile shapes by line length: rows of at most
+                           [0] cells use 64x16 tiles, of at most [1] cells 16x64,
+                           longer rows 4x256; columns likewise with [2] and [3],
+                           longer columns 16x256. 0 selects the defaults 24, 160,
+                           24, 384. They change the rounding of the line solves,
+                           so results are reproducible for the same values. */
+  int reserved[2];
 } sg_opts;
 
 typedef struct {

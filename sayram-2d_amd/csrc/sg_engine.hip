@@ -138,10 +138,10 @@ struct Maps {
 // whose loads fit the registers of 1024-thread tiles only in FP32. It depends
 // only on the instance, its level and the batch's precision, so results do
 // not depend on the rest of the batch.
-static int tile_shape(int n, int dir, int l, bool fp64_vcycle) {
-  if (n <= 24) return 0;
-  if (dir == 0) return n <= 160 ? 1 : 2;
-  return (n <= 384 || (fp64_vcycle && l > 0)) ? 1 : 3;
+static int tile_shape(const int (&lim)[4], int n, int dir, int l, bool fp64_vcycle) {
+  if (n <= lim[2 * dir]) return 0;
+  if (dir == 0) return n <= lim[1] ? 1 : 2;
+  return (n <= lim[3] || (fp64_vcycle && l > 0)) ? 1 : 3;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
@@ -260,8 +260,11 @@ static void apply_defaults(sg_opts &o) {
   if (o.max_iters <= 0) o.max_iters = o.precision == SG_MIXED ? 12 : 200;
   if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
   if (o.max_inner <= 0) o.max_inner = 40;
-  if (o.check_every <= 0) o.check_every = 2;
+  if (o.check_every <= 0) o.check_every = 1;
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
+  const int tiles[4] = {24, 160, 24, 384};
+  for (int k = 0; k < 4; ++k)
+    if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
 }
 
 int Batch::init(const sg_opts *o, size_t b) {
@@ -505,14 +508,14 @@ int Batch::rebuild() {
       }
       if (coarsest) {
         if (ly == 1)
-          M.gcx[l][tile_shape(lx, 0, l, f64)].push_back({c, 0, 1, 1});
+          M.gcx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)].push_back({c, 0, 1, 1});
         else
-          M.gcy[l][tile_shape(ly, 1, l, f64)].push_back({c, 0, 1, 1});
+          M.gcy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)].push_back({c, 0, 1, 1});
       } else {
         // groups of tile_lines(shape) lines of one colour
         for (int d = 0; d < 2; ++d) {
           const int nl = d == 0 ? ly : lx;          // number of lines
-          const int sh = tile_shape(d == 0 ? lx : ly, d, l, f64);
+          const int sh = tile_shape(opts.line_tiles, d == 0 ? lx : ly, d, l, f64);
           const int per = tile_lines(sh);
           for (int p = 0; p < 2; ++p)
             for (int f = p; f < nl; f += 2 * per) {
@@ -649,11 +652,6 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      // algorithmic traffic of a level-0 sweep: per swept cell b, the lower
-      // band, the two line factors and the two off-line couplings are read,
-      // one value of x of the other colour is read (each is shared by the two
-      // lines beside it) and x is written: 8 values
-      if (lid == time_id && l == 0) t_bytes += 8.0 * sizeof(T) * m.gcells[l][dir][p];
       tbegin(lid);
       for (int sh = 0; sh < kNumTileShapes; ++sh)
         launch_line<T>(stream, d_insts, m.gmap[l][dir][p][sh], m.ngmap[l][dir][p][sh], l, dir, sh,
@@ -824,6 +822,20 @@ int Batch::step_once() {
                         stream));
   HIPCHK(hipStreamSynchronize(stream));
   if ((rc = collect_time())) return rc;
+  if (time_id == 2) {
+    // Algorithmic traffic of the timed level-0 sweeps, from the V-cycles each
+    // instance ran (a converged instance's work-groups exit at once): per
+    // swept cell b, the lower band, the two line factors and the two off-line
+    // couplings are read, one value of x of the other colour is read (each is
+    // shared by the two lines beside it) and x is written: 8 values. A V-cycle
+    // sweeps every cell twice (rows and columns, each in two colours).
+    const double vt = opts.precision == SG_FP64 ? sizeof(double) : sizeof(float);
+    for (int c = 0; c < n; ++c) {
+      if (!part[c]) continue;
+      const Slot &s = slots[order[c]];
+      t_bytes += h_scal[order[c]].vcyc * 2.0 * 8.0 * vt * (double)s.nx * s.ny;
+    }
+  }
   int worst = SG_OK;
   for (int c = 0; c < n; ++c) {
     if (!part[c]) continue;

@@ -80,7 +80,8 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   int ostatus, orefs;    // mixed: outer refinement status and count
   int maxit, clamps;
   int omaxit, part;
-  int titers, pad;      // Krylov iterations summed over the refinements of a step
+  int titers;           // Krylov iterations summed over the refinements of a step
+  int vcyc;             // level-0 V-cycles this instance ran in the step
 };
 
 struct Lev {            // one multigrid level of one instance

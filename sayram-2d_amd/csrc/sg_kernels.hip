@@ -122,7 +122,7 @@ __global__ void k_begin(const InstDev *__restrict__ insts, int n, const int *__r
     s.status = p ? ST_ACTIVE : ST_IDLE;
     s.ostatus = p ? ST_ACTIVE : ST_IDLE;
     s.tol2 = tol2; s.otol2 = otol2; s.maxit = maxit; s.omaxit = omaxit;
-    s.iters = 0; s.titers = 0; s.orefs = 0; s.clamps = 0;
+    s.iters = 0; s.titers = 0; s.orefs = 0; s.clamps = 0; s.vcyc = 0;
     mine += p;
   }
   atomicAdd(&cnt, mine);
@@ -1312,10 +1312,10 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
       s.pcs = t[0] > 0.0 ? sqrt(t[0]) : 1.0;
       if (t[0] <= s.tol2 * t[1]) { s.status = ST_DONE; atomicSub(cnt, 1); }
     }
-  } else if (MODE == F_V) {
+  } else if (MODE == F_V) {   // after the V-cycle of p (active instances only)
     T t[1];
     sum_partials<T, 1>(I, t);
-    if (threadIdx.x == 0) s.alpha = (double)((T)s.rho / t[0]);
+    if (threadIdx.x == 0) { s.alpha = (double)((T)s.rho / t[0]); s.vcyc += 1; }
   } else if (MODE == F_S) {
     T t[1];
     sum_partials<T, 1>(I, t);
@@ -1323,10 +1323,10 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
       s.status = ST_HALF;
       s.rnorm2 = (double)t[0];
     }
-  } else if (MODE == F_T) {
+  } else if (MODE == F_T) {   // after the V-cycle of s (still-active instances only)
     T t[2];
     sum_partials<T, 2>(I, t);
-    if (threadIdx.x == 0) s.omega = t[1] > T(0) ? (double)(t[0] / t[1]) : 0.0;
+    if (threadIdx.x == 0) { s.omega = t[1] > T(0) ? (double)(t[0] / t[1]) : 0.0; s.vcyc += 1; }
   } else if (MODE == F_X) {
     T t[2] = {T(0), T(0)};
     if (st == ST_ACTIVE) sum_partials<T, 2>(I, t);

@@ -1,0 +1,87 @@
+"""Autotuner of the batched implicit step's launch parameters, by measurement.
+
+On a caller's workload (a list of problems: the production mix, or a sample of
+it) the tuner times the batched step for candidate values of the parameters
+that select how the step is launched, and returns the fastest set:
+
+* ``check_every``: Krylov iterations between host convergence checks. A larger
+  value issues fewer host synchronisations; a smaller one runs fewer batched
+  iterations after the last instance has converged.
+* ``use_graph``: Krylov chunks replayed from captured hipGraphs (1) or launched
+  eagerly (-1).
+* ``line_tiles``: the line lengths at which the line solves switch tile shape
+  (rows: short / medium limits; columns: short / medium limits).
+
+``check_every`` and ``use_graph`` never change results: every instance's
+result is bit-identical for all their values. ``line_tiles`` changes the
+rounding of the line solves (not their accuracy), so the tuned values are part
+of the options a run is reproduced with.
+
+Each trial builds a fresh batch, so no state is carried between trials. It runs
+``warmup`` steps, then times ``steps`` steps. The search is coordinate descent
+from the library defaults, with every measured point cached.
+"""
+from __future__ import annotations
+
+import time
+
+import sayram_gpu as sg
+
+DEFAULTS = {"check_every": 1, "use_graph": 1, "row_short": 24, "row_mid": 160,
+            "col_short": 24, "col_mid": 384}
+
+CANDIDATES = {
+    "check_every": [1, 2, 3, 4],
+    "use_graph": [1, -1],
+    "row_mid": [96, 160, 256],
+    "col_mid": [256, 384, 768],
+    "row_short": [16, 24, 32],
+    "col_short": [16, 24, 32],
+}
+
+
+def opts_for(precision: int, p: dict):
+    return sg.make_opts(precision=precision, check_every=p["check_every"], use_graph=p["use_graph"],
+                        line_tiles=(p["row_short"], p["row_mid"], p["col_short"], p["col_mid"]))
+
+
+def measure(problems, precision: int, p: dict, steps: int = 2, warmup: int = 1) -> float:
+    """Cell-steps per second of the batched step with parameters p."""
+    b = sg.Batch(opts_for(precision, p))
+    try:
+        for q in problems:
+            b.add(q)
+        b.step(warmup)
+        t0 = time.perf_counter()
+        b.step(steps)
+        dt = time.perf_counter() - t0
+    finally:
+        b.close()
+    return sum(q.ncells for q in problems) * steps / dt
+
+
+def tune(problems, precision: int = sg.SG_PC32, steps: int = 2, warmup: int = 1,
+         order=("check_every", "use_graph", "col_mid", "row_mid", "col_short", "row_short"),
+         log=None):
+    """Coordinate descent from DEFAULTS. Returns (best parameters, best value,
+    list of (parameters, cell-steps/s) for every measured point)."""
+    cache = {}
+
+    def value(p):
+        key = tuple(sorted(p.items()))
+        if key not in cache:
+            cache[key] = measure(problems, precision, p, steps, warmup)
+            if log:
+                log(f"[tune] {p} -> {cache[key]:.4e} cell-steps/s")
+        return cache[key]
+
+    best = dict(DEFAULTS)
+    best_v = value(best)
+    for name in order:
+        for cand in CANDIDATES[name]:
+            p = dict(best, **{name: cand})
+            v = value(p)
+            if v > best_v:
+                best, best_v = p, v
+    trials = [(dict(k), v) for k, v in cache.items()]
+    return best, best_v, trials

@@ -100,3 +100,32 @@ def test_batch_specs():
     assert all(32 <= x["nx"] <= 512 and 32 <= x["ny"] <= 512 for x in s)
     assert all(0.1 <= x["scale"] <= 10.0 for x in s)
     assert s == sc.batch_specs(512, 32, 512, 2026)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of sg_opts and sg_step_stats have the C layout: a
+    program compiled against include/sayram_gpu.h prints sizes and offsets."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = {"sg_opts": [f for f, _ in sg.SgOpts._fields_],
+              "sg_step_stats": [f for f, _ in sg.SgStats._fields_]}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sayram_gpu.h"', 'int main(void) {']
+    for t, fs in fields.items():
+        lines.append(f'printf("{t} %zu\\n", sizeof({t}));')
+        for f in fs:
+            lines.append(f'printf("{t}.{f} %zu\\n", offsetof({t}, {f}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                          check=True).stdout.split("\n") if l)
+    for t, cls in (("sg_opts", sg.SgOpts), ("sg_step_stats", sg.SgStats)):
+        assert int(out[t]) == ctypes.sizeof(cls), t
+        for f in fields[t]:
+            assert int(out[f"{t}.{f}"]) == getattr(cls, f).offset, f"{t}.{f}"
