@@ -47,6 +47,7 @@ PREC_NAMES = {"pc32": "FP64 BiCGSTAB with an FP32 multigrid preconditioner",
               "mixed": "FP32 BiCGSTAB inside FP64 iterative refinement",
               "fp64": "FP64 throughout"}
 HBM_PEAK_GBS = 8000.0
+KERNEL_DESC = "k_line<T,5,*,false,*> level-0 zebra line sweep dispatches (every tile shape)"
 
 
 def log(*a):
@@ -242,6 +243,10 @@ def main():
         f"cell-steps/s, Krylov iterations mean {np.mean(iters):.1f} max {max(iters)}, "
         f"refinements mean {np.mean(refs):.2f}")
     if args.profile_only:
+        if kl and kms > 0:
+            log(f"[bench] {KERNEL_DESC}: {kl} dispatches, HIP-event mean {kms / kl * 1e3:.1f} us, "
+                f"{kbytes / kl / 1e6:.1f} MB algorithmic per dispatch, "
+                f"{kbytes / (kms * 1e-3) / 1e9:.0f} GB/s")
         return
 
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
@@ -272,8 +277,7 @@ def main():
     if kl and kms > 0:
         avg_ms = kms / kl
         achieved = kbytes / (kms * 1e-3) / 1e9
-        out["roofline"] = {"kernel": "k_line<float,5,*,false,*> level-0 zebra line sweeps (all "
-                                     "instances, every tile shape)",
+        out["roofline"] = {"kernel": KERNEL_DESC,
                            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_launch_ms": avg_ms, "launches": kl,

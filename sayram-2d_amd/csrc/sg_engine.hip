@@ -652,11 +652,13 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      tbegin(lid);
-      for (int sh = 0; sh < kNumTileShapes; ++sh)
+      for (int sh = 0; sh < kNumTileShapes; ++sh) {
+        if (!m.ngmap[l][dir][p][sh]) continue;
+        tbegin(lid);   // one timed interval per kernel dispatch
         launch_line<T>(stream, d_insts, m.gmap[l][dir][p][sh], m.ngmap[l][dir][p][sh], l, dir, sh,
                        zero, xid, bid);
-      tend(lid);
+        tend(lid);
+      }
       zero = false;
     }
   };

@@ -1,6 +1,15 @@
 // sayram_gpu_run -- the reference's driver (source/main.cc) on the gfx950 step.
 //
-//   sayram_gpu_run <p.ini> <problem.bin>
+//   sayram_gpu_run [--case albert_young|albert_young_lc] [--dump-case <file>] <p.ini>
+//   sayram_gpu_run [--dump-case <file>] <p.ini> <problem.bin>
+//
+// Without problem.bin the Equation is the reference's Albert & Young chorus
+// case (source/main.cc:54 constructs Albert_Young; albert_young_lc is the
+// loss-cone variant of source/Cases/Albert_Young_LC.cc), built from the
+// diffusion table D/<dID>.h5 of the working directory as
+// source/Cases/Albert_Young_IO.cc:19 opens it ([diffusion_coefficients] dID).
+// --dump-case writes the Equation's inputs in problem.bin format and exits
+// without touching the GPU.
 //
 // p.ini is read with the reference's keys and derived values
 // (source/Parameters.cc:35-65): [basic] run_id, nalpha0, nE, alpha0min,
@@ -25,12 +34,14 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
 #include <vector>
 
 #include "GpuSolver.h"
+#include "albert_young.h"
 #include "sayram_host.h"
 
 #ifdef SAYRAM_HDF5
@@ -79,6 +90,7 @@ struct Params {
   std::size_t nalpha0 = 0, nE = 0;
   double a0min = 0, a0max = 0, Emin = 0, Emax = 0, T = 0, nsteps = 0;
   int nplots = 1, save_every = 1;
+  std::string dID;
   double dt() const { return T / nsteps; }
 };
 
@@ -102,6 +114,8 @@ Params read_params(const std::string &path) {
   p.T = std::stod(get("basic", "t"));
   p.nsteps = std::stod(get("basic", "nsteps"));
   p.nplots = std::stoi(get("diagnostics", "nplots"));
+  auto dc = ini.find("diffusion_coefficients");
+  if (dc != ini.end() && dc->second.count("did")) p.dID = dc->second.at("did");
   // source/Parameters.cc:59-60
   p.save_every = (int)(p.nsteps / p.nplots);
   p.nsteps = p.save_every * p.nplots;
@@ -169,6 +183,46 @@ class ArrayCase : public Equation {
 };
 
 #ifdef SAYRAM_HDF5
+std::vector<double> h5_read(hid_t file, const char *name, std::vector<hsize_t> &dims) {
+  hid_t ds = H5Dopen2(file, name, H5P_DEFAULT);
+  if (ds < 0) throw std::runtime_error(std::string("HDF5: no dataset ") + name);
+  hid_t space = H5Dget_space(ds);
+  const int rank = H5Sget_simple_extent_ndims(space);
+  dims.assign(rank > 0 ? rank : 0, 0);
+  if (rank > 0) H5Sget_simple_extent_dims(space, dims.data(), nullptr);
+  hsize_t n = 1;
+  for (hsize_t d : dims) n *= d;
+  std::vector<double> v(n);
+  const herr_t rc = H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, v.data());
+  H5Sclose(space);
+  H5Dclose(ds);
+  if (rc < 0) throw std::runtime_error(std::string("HDF5: cannot read ") + name);
+  return v;
+}
+
+// D/<dID>.h5 as source/Cases/Albert_Young_IO.cc:17-37 reads it
+ay::DTable read_d_table(const std::string &path) {
+  hid_t f = H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) throw std::runtime_error("cannot open " + path);
+  ay::DTable t;
+  std::vector<hsize_t> da, de, d2;
+  t.x = h5_read(f, "/alpha0", da);
+  for (double &v : t.x) v = v * ay::kPi / 180;    // degrees to radians
+  t.y = h5_read(f, "/E", de);
+  t.daa = h5_read(f, "/Daa", d2);
+  const auto check = [&](const std::vector<hsize_t> &d, const char *nm) {
+    if (d.size() != 2 || d[0] != t.x.size() || d[1] != t.y.size())
+      throw std::runtime_error(std::string("D table: ") + nm + " is not (n_alpha0, n_E)");
+  };
+  check(d2, "/Daa");
+  t.dap = h5_read(f, "/Dap", d2);
+  check(d2, "/Dap");
+  t.dpp = h5_read(f, "/Dpp", d2);
+  check(d2, "/Dpp");
+  H5Fclose(f);
+  return t;
+}
+
 void h5_write(hid_t file, const std::string &name, const double *v, int rank, const hsize_t *dims) {
   hid_t space = H5Screate_simple(rank, dims, nullptr);
   hid_t lcpl = H5Pcreate(H5P_LINK_CREATE);
@@ -182,18 +236,81 @@ void h5_write(hid_t file, const std::string &name, const double *v, int rank, co
 }
 #endif
 
+// the Equation's inputs in problem.bin format (sayram_cases.write_problem)
+void dump_case(const Mesh &m, const Equation &eq, const std::string &path) {
+  std::ofstream fo(path, std::ios::binary);
+  if (!fo) throw std::runtime_error("cannot write " + path);
+  const std::size_t nx = m.nx(), ny = m.ny();
+  fo.write("SGPROB1", 8);
+  const std::int32_t n[2] = {(std::int32_t)nx, (std::int32_t)ny};
+  fo.write(reinterpret_cast<const char *>(n), sizeof(n));
+  std::vector<double> a(nx * ny);
+  auto put = [&](auto get) {
+    for (std::size_t i = 0; i < nx; ++i)
+      for (std::size_t j = 0; j < ny; ++j) a[i * ny + j] = get(Ind{i, j});
+    fo.write(reinterpret_cast<const char *>(a.data()), (std::streamsize)(a.size() * sizeof(double)));
+  };
+  put([&](const Ind &k) { return eq.G(k); });
+  put([&](const Ind &k) { return eq.Dxx(k); });
+  put([&](const Ind &k) { return eq.Dxy(k); });
+  put([&](const Ind &k) { return eq.Dyy(k); });
+  put([&](const Ind &k) { return eq.inv_tau(k); });
+  put([&](const Ind &k) { return eq.init_f(k); });
+  const BoundaryID sides[4] = {BoundaryID::XMIN, BoundaryID::XMAX, BoundaryID::YMIN, BoundaryID::YMAX};
+  std::int32_t bt[4];
+  for (int s = 0; s < 4; ++s) bt[s] = eq.bc_type(sides[s]) == BCType::Dirichlet ? 0 : 1;
+  fo.write(reinterpret_cast<const char *>(bt), sizeof(bt));
+  for (int s = 0; s < 4; ++s) {
+    const std::size_t nv = s < 2 ? ny + 1 : nx + 1;
+    std::vector<double> v(nv, 0.0);
+    for (std::size_t k = 0; k < nv; ++k) {
+      double out = 0.0;
+      if (bt[s] == 0 && eq.dirichlet_vertex_value(sides[s], s < 2 ? (s == 0 ? 0 : nx) : k,
+                                                  s < 2 ? k : (s == 2 ? 0 : ny), 0.0, &out))
+        v[k] = out;
+    }
+    fo.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(nv * sizeof(double)));
+  }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc != 3) {
-    std::cerr << "usage: sayram_gpu_run <p.ini> <problem.bin>" << std::endl;
+  std::string case_name = "albert_young", dump;
+  std::vector<std::string> pos;
+  for (int k = 1; k < argc; ++k) {
+    const std::string a = argv[k];
+    if (a == "--case" && k + 1 < argc) case_name = argv[++k];
+    else if (a == "--dump-case" && k + 1 < argc) dump = argv[++k];
+    else pos.push_back(a);
+  }
+  if (pos.empty() || pos.size() > 2 ||
+      (case_name != "albert_young" && case_name != "albert_young_lc")) {
+    std::cerr << "usage: sayram_gpu_run [--case albert_young|albert_young_lc] [--dump-case <file>] "
+                 "<p.ini> [<problem.bin>]" << std::endl;
     return 1;
   }
   try {
-    const Params paras = read_params(argv[1]);
+    const Params paras = read_params(pos[0]);
     Grid2D grid = make_uniform(paras);
     Mesh m(grid, paras.dt());
-    ArrayCase eq(m, argv[2]);
+    std::unique_ptr<Equation> eqp;
+    if (pos.size() == 2) {
+      eqp.reset(new ArrayCase(m, pos[1]));
+    } else {
+#ifdef SAYRAM_HDF5
+      if (paras.dID.empty()) throw std::runtime_error("[diffusion_coefficients] dID not set");
+      const ay::DTable tab = read_d_table("D/" + paras.dID + ".h5");
+      eqp.reset(new ay::AlbertYoung(m, tab, std::log(paras.Emin), case_name == "albert_young_lc"));
+#else
+      throw std::runtime_error("the Albert_Young case reads its HDF5 table: build with HDF5");
+#endif
+    }
+    Equation &eq = *eqp;
+    if (!dump.empty()) {
+      dump_case(m, eq, dump);
+      return 0;
+    }
     GpuSolver solver(m, &eq);
     const std::string out_dir = "./output/" + paras.run_id + "/";
     ::mkdir("./output", 0755);

@@ -143,3 +143,114 @@ def write_problem(p: Problem, path: str) -> None:
         for s, n in zip(range(4), (p.ny + 1, p.ny + 1, p.nx + 1, p.nx + 1)):
             v = p.bc_val[s] if p.bc_val[s] is not None else np.zeros(n)
             fh.write(np.ascontiguousarray(v, dtype=np.float64).tobytes())
+
+
+# ---------------------------------------------------------------------------
+# The reference's Albert & Young (2005) chorus cases (source/Cases/
+# Albert_Young.{h,cc}, Albert_Young_LC.{h,cc}) from the diffusion table
+# D/<dID>.h5 (source/Cases/Albert_Young_IO.cc:17-37). Every value is evaluated
+# per cell with the math module (the platform libm) in the reference's
+# operation order, like host/albert_young.h, so both give identical inputs.
+# ---------------------------------------------------------------------------
+RE_EARTH = 6371000.0      # m, source/common.h:44
+EPS = 2.0 ** -52          # gEPS, source/common.h:38
+
+
+def _e2p(E):              # source/utils.h:17-19, gC = 1
+    return math.sqrt(E * (E + 2 * E0)) / 1.0
+
+
+def _p2e(p):              # source/utils.h:12-14
+    return math.sqrt(p * p * 1.0 * 1.0 + E0 * E0) - E0
+
+
+def _dlogE_dp(logE):      # source/utils.h:21-24
+    E = math.exp(logE)
+    return _e2p(E) * 1.0 * 1.0 / (E * (E + E0))
+
+
+def _cal_weight(pos, n):
+    """calWeight (source/utils.h:41-51) with the reference's size_t index: a
+    position below the table becomes an out-of-range unsigned index on x86-64
+    and is clamped to the last interval with weight 0."""
+    fl = math.floor(pos)
+    i = n if fl < 0 else int(fl)
+    if i < n:
+        return i, 1.0 - (pos - float(i))
+    return n - 1, 0.0
+
+
+def _interp2d(raw, i0, j0, wi, wj):   # interp2D, source/utils.h:25-38
+    return (raw[i0][j0] * wi * wj + raw[i0 + 1][j0] * (1 - wi) * wj +
+            raw[i0 + 1][j0 + 1] * (1 - wi) * (1 - wj) + raw[i0][j0 + 1] * wi * (1 - wj))
+
+
+def albert_young_case(nalpha0, nE, alpha0min, alpha0max, Emin, Emax, dt, table, lc=False):
+    """Problem of the Albert_Young (lc=False) or Albert_Young_LC (lc=True)
+    Equation on make_uniform's grid. table: dict with 'alpha0' (degrees), 'E'
+    (MeV) and 'Daa', 'Dap', 'Dpp' of shape (n_alpha0, n_E), as stored in
+    D/<dID>.h5."""
+    xe = uniform_edges(nalpha0, alpha0min * GPI / 180, alpha0max * GPI / 180)
+    logEmin, logEmax = math.log(Emin), math.log(Emax)
+    ye = uniform_edges(nE, logEmin, logEmax)
+    x = [0.5 * (float(xe[i]) + float(xe[i + 1])) for i in range(nalpha0)]
+    y = [0.5 * (float(ye[j]) + float(ye[j + 1])) for j in range(nE)]
+    tx = [float(v) * GPI / 180 for v in np.asarray(table["alpha0"], dtype=np.float64)]
+    ty = [float(v) for v in np.asarray(table["E"], dtype=np.float64)]
+    raw = {k: np.asarray(table[k], dtype=np.float64).tolist() for k in ("Daa", "Dap", "Dpp")}
+    xmin, xmax, ymin, ymax = tx[0], tx[-1], ty[0], ty[-1]
+
+    def init_f(a, logE):     # Albert_Young.h:37-40, Albert_Young_LC.h:36-39
+        p = _e2p(math.exp(logE))
+        if lc:
+            return math.exp(-(math.exp(logE) - 0.2) / 0.1) * math.sin(a) / (p * p) + EPS
+        return (math.exp(-(math.exp(logE) - 0.2) / 0.1) * (math.sin(a) - math.sin(5 * GPI / 180)) /
+                (p * p) + EPS)
+
+    def calc_G(alpha, logE):   # Albert_Young.h:42-45
+        t = 1.30 - 0.56 * math.sin(alpha)
+        return (math.pow(_e2p(math.exp(logE)), 2) * t * math.sin(alpha) * math.cos(alpha) /
+                _dlogE_dp(logE))
+
+    G = np.empty((nalpha0, nE)); f0 = np.empty((nalpha0, nE))
+    Dxx = np.empty((nalpha0, nE)); Dxy = np.empty((nalpha0, nE)); Dyy = np.empty((nalpha0, nE))
+    itau = np.zeros((nalpha0, nE))
+    denorm = (E0 / (1.0 * 1.0)) * (E0 / (1.0 * 1.0)) * 1.0 * 1.0
+    s2d = float(3600 * 24)
+    for i in range(nalpha0):
+        for j in range(nE):
+            G[i, j] = calc_G(x[i], y[j])
+            f0[i, j] = init_f(x[i], y[j])
+    if lc:                     # Albert_Young_LC.cc:42-53, bounce_period :150-160
+        L = 4.5
+        alpha0lc = math.asin(math.pow(math.pow(L, 5) * (4 * L - 3), -0.25))
+        T0, T1 = 1.3802, 0.7405
+        for i in range(nalpha0):
+            a0 = x[i]
+            if a0 < alpha0lc:
+                for j in range(nE):
+                    p = _e2p(math.exp(y[j]))
+                    s = math.sin(a0)
+                    Ty = T0 - 0.5 * (T0 - T1) * (s + math.sqrt(s))
+                    tb = 4 * L * RE_EARTH * ((E0 + _p2e(p)) / (1.0 * 1.0)) / p * Ty / (3e8 * 3600 * 24)
+                    itau[i, j] = 4.0 / tb
+    for i in range(nalpha0):   # constructD, Albert_Young.cc:113-134
+        a0 = x[i]
+        for j in range(nE):
+            logE = y[j]
+            p = _e2p(math.exp(logE))
+            pos_x = (a0 - xmin) / (xmax - xmin) * float(len(tx) - 1)
+            pos_y = (logE - math.log(ymin)) / (math.log(ymax) - math.log(ymin)) * float(len(ty) - 1)
+            i0, wi = _cal_weight(pos_x, len(tx) - 1)
+            j0, wj = _cal_weight(pos_y, len(ty) - 1)
+            dl = _dlogE_dp(logE)
+            Dxx[i, j] = _interp2d(raw["Daa"], i0, j0, wi, wj) * denorm * s2d / (p * p)
+            Dxy[i, j] = _interp2d(raw["Dap"], i0, j0, wi, wj) * denorm * s2d * dl / p
+            Dyy[i, j] = _interp2d(raw["Dpp"], i0, j0, wi, wj) * denorm * s2d * math.pow(dl, 2)
+    # boundaries (Albert_Young.cc:42-92): XMIN Dirichlet 0 (zero flux for LC),
+    # XMAX zero flux, YMIN Dirichlet init_f(x_edge, log Emin), YMAX Dirichlet 0
+    bc_type = [ZEROFLUX if lc else DIRICHLET, ZEROFLUX, DIRICHLET, DIRICHLET]
+    bc_val = [None if lc else np.zeros(nE + 1), None,
+              np.array([init_f(float(v), logEmin) for v in xe]), np.zeros(nalpha0 + 1)]
+    return Problem(nalpha0, nE, xe, ye, G, Dxx, Dxy, Dyy, itau, bc_type, bc_val, f0, float(dt),
+                   dict(case="albert_young_lc" if lc else "albert_young"))

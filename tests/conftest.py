@@ -30,3 +30,27 @@ def gpu_lib():
     if L.sg_device_count() < 1:
         pytest.fail("no GPU visible to the HIP runtime")
     return sg
+
+
+H5DUMP = None
+for _c in ("h5dump", "/opt/conda/bin/h5dump"):
+    import shutil as _sh
+    _p = _sh.which(_c) or (_c if os.path.exists(_c) else None)
+    if _p:
+        H5DUMP = _p
+        break
+
+
+def read_h5(path, name):
+    """A dataset of an HDF5 file as float64, through h5dump at 17 significant
+    digits (round-trips every double); shape from the dataspace."""
+    import re
+    import subprocess
+    if H5DUMP is None:
+        pytest.skip("h5dump not available")
+    out = subprocess.run([H5DUMP, "-d", name, "-m", "%.17g", "-y", "-w", "0", path],
+                         capture_output=True, text=True, check=True).stdout
+    dims = [int(v) for v in re.search(r"DATASPACE\s+SIMPLE \{ \( ([^)]*) \)", out).group(1).split(",")]
+    body = out.split("DATA {", 1)[1].split("}", 1)[0]
+    vals = np.array([float(v) for v in body.replace(",", " ").split()], dtype=np.float64)
+    return vals.reshape(dims)
