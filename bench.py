@@ -282,6 +282,16 @@ def main():
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_launch_ms": avg_ms, "launches": kl,
                            "algorithmic_bytes_per_launch": kbytes / kl}
+        # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # runs recorded beside the kernel-trace summary
+        tf = os.path.join(ROOT, "profiles", "line_l0_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                t = json.load(fh)
+            out["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
+            out["roofline"]["traffic_basis"] = (
+                f"{t['basis']}; {t['traffic_bytes_per_dispatch'] / t['algorithmic_bytes_per_dispatch']:.2f}x "
+                f"the algorithmic bytes there ({t['summary']})")
 
     if not args.skip_precisions:
         out["precisions"] = {}

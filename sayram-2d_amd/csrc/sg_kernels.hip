@@ -782,16 +782,19 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       if (sk_ok && sc == W - 1) s_c[sk] = d;
     }
     lds_barrier();
+    // The last tile's d stays in LDS: the backward pass starts there.
+    if (t0 + S < n) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int k = ek(q), u = eu(q);
-      if (k < cnt && u < tw)
-        st32(x, at(gr.first + k * gr.step, t0 + u), s_r[tslot<S>(k, u)]);
+      for (int q = 0; q < Q; ++q) {
+        const int k = ek(q), u = eu(q);
+        if (k < cnt && u < tw)
+          st32(x, at(gr.first + k * gr.step, t0 + u), s_r[tslot<S>(k, u)]);
+      }
     }
     lds_barrier();
   }
   T vd[Q], vc[Q];
-  auto load_bwd = [&](int t0) {
+  auto load_bwd = [&](int t0, bool with_d) {
     const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -799,25 +802,25 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       vd[q] = vc[q] = T(0);
       if (k < cnt && u < tw) {
         const uint32_t K = at(gr.first + k * gr.step, t0 + u);
-        vd[q] = ld32(x, K);
+        if (with_d) vd[q] = ld32(x, K);
         vc[q] = ld32(cu, K);
       }
     }
   };
   if (tid < L) s_c[tid] = T(0);
   const int ntiles = (n + S - 1) / S;
-  // d of every tile was stored above by the thread that loads it here
-  load_bwd((ntiles - 1) * S);
+  // d of every earlier tile was stored above by the thread that loads it here
+  load_bwd((ntiles - 1) * S, false);
   for (int tt = ntiles - 1; tt >= 0; --tt) {
     const int t0 = tt * S;
     const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      s_r[tslot<S>(ek(q), eu(q))] = vd[q];
+      if (tt < ntiles - 1) s_r[tslot<S>(ek(q), eu(q))] = vd[q];
       s_a[tslot<S>(ek(q), eu(q))] = vc[q];
     }
     // prefetch the tile below (this iteration stores tile tt only)
-    if (tt > 0) load_bwd(t0 - S);
+    if (tt > 0) load_bwd(t0 - S, true);
     lds_barrier();
     {   // backward: x_t = -u_t x_{t+1} + d_t, composed from the line's end
       T al[4], be[4];
