@@ -96,7 +96,13 @@ typedef struct {
                            longer columns 16x256. 0 selects the defaults 24, 160,
                            24, 384. They change the rounding of the line solves,
                            so results are reproducible for the same values. */
-  int reserved[2];
+  int fuse_cells;       /* the V-cycle levels of an instance from its first level
+                           of at most fuse_cells cells down run in one fused
+                           launch per instance (post-smoothing schedules);
+                           0 selects 1024, the maximum (a level is kept in
+                           LDS); -1 disables. Like line_tiles it changes the
+                           rounding of those line solves. */
+  int reserved[1];
 } sg_opts;
 
 typedef struct {

@@ -115,6 +115,7 @@ struct Slot {
   bool used = false;
   int nx = 0, ny = 0;
   int prec = SG_FP64;
+  int fuse_lev = 0;       // first level of the fused coarse cycle (nlev: none)
   void *slab = nullptr;
   size_t bytes = 0;
   InstDev dev{};
@@ -124,10 +125,19 @@ struct Slot {
   sg_step_stats st{};
 };
 
+// Setup maps cover every level of every instance (assembly, Galerkin chain,
+// line factors). The cycle maps leave out the levels an instance runs in the
+// fused coarse cycle.
 struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
   std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
       gcy[kMaxLev][kNumTileShapes];
+  // cycle: restriction into level l, prolongation into level l, sweeps and
+  // coarsest solves, and every instance with a fused coarse cycle
+  std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev];
+  std::vector<LineGrp> sw[kMaxLev][2][2][kNumTileShapes], swx[kMaxLev][kNumTileShapes],
+      swy[kMaxLev][kNumTileShapes];
+  std::vector<FuseMap> fmap;
 };
 
 // Tile shape of a line solve by line length and direction: short lines in
@@ -168,7 +178,14 @@ struct DevMaps {   // offsets into one device buffer
       *gcy[kMaxLev][kNumTileShapes];
   int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
       ngcy[kMaxLev][kNumTileShapes];
-  double gcells[kMaxLev][2][2];   // cells swept by one launch of a smoothing map
+  const BlkMap *rmap[kMaxLev], *qmap[kMaxLev];
+  int nrmap[kMaxLev], nqmap[kMaxLev];
+  const LineGrp *sw[kMaxLev][2][2][kNumTileShapes], *swx[kMaxLev][kNumTileShapes],
+      *swy[kMaxLev][kNumTileShapes];
+  int nsw[kMaxLev][2][2][kNumTileShapes], nswx[kMaxLev][kNumTileShapes],
+      nswy[kMaxLev][kNumTileShapes];
+  const FuseMap *fmap;
+  int nfmap;
   int maxlev;
 };
 
@@ -223,6 +240,7 @@ struct Batch {
   int upload_bc(Slot &s, int side, const double *vals);
   int remove(int slot);
   int rebuild();
+  int fuse_level(const InstDev &I) const;
   int step(int nsteps, const int64_t *targets, int ntargets);
   int step_once();
   // KT: Krylov vectors, VT: V-cycle (KT double with VT float is SG_PC32)
@@ -262,6 +280,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 1;
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
+  if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   const int tiles[4] = {24, 160, 24, 384};
   for (int k = 0; k < 4; ++k)
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
@@ -272,6 +291,8 @@ int Batch::init(const sg_opts *o, size_t b) {
   if (opts.precision != SG_FP64 && opts.precision != SG_MIXED && opts.precision != SG_PC32)
     return fail(SG_ERR_ARG, "opts.precision must be SG_FP64, SG_MIXED or SG_PC32");
   apply_defaults(opts);
+  if (opts.fuse_cells > kFuseCells)
+    return fail(SG_ERR_ARG, "opts.fuse_cells must be at most 1024 (the fused cycle keeps a level in LDS)");
   budget = b;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -480,6 +501,17 @@ int Batch::remove(int slot) {
   return SG_OK;
 }
 
+// First level of an instance's fused coarse cycle: its first level with at
+// most fuse_cells cells; nlev (none) when fusion is off, when the schedule
+// pre-smooths (the fused cycle implements post-smoothing) or when no level is
+// that small. It depends only on the instance and the options.
+int Batch::fuse_level(const InstDev &I) const {
+  if (opts.fuse_cells < 0 || (opts.smoother & 3)) return I.nlev;
+  for (int l = 0; l < I.nlev; ++l)
+    if ((long long)I.lev[l].nx * I.lev[l].ny <= opts.fuse_cells) return l;
+  return I.nlev;
+}
+
 void Batch::drop_graphs() {
   for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
   graphs.clear();
@@ -499,18 +531,26 @@ int Batch::rebuild() {
     const InstDev &I = slots[k].dev;
     table.push_back(I);
     maxlev = std::max(maxlev, I.nlev);
+    const int lf = fuse_level(I);
+    slots[k].fuse_lev = lf;
+    if (lf < I.nlev) M.fmap.push_back({c, lf});
     for (int l = 0; l < I.nlev; ++l) {
       const int lx = I.lev[l].nx, ly = I.lev[l].ny, Nl = lx * ly;
       const bool coarsest = (l == I.nlev - 1);
       for (int k0 = 0; k0 < Nl; k0 += kBlock) {
         M.cmap[l].push_back({c, k0});
         if (!coarsest) M.pmap[l].push_back({c, k0});
+        if (l >= 1 && l <= lf) M.rmap[l].push_back({c, k0});        // restricted into from l - 1
+        if (!coarsest && l < lf) M.qmap[l].push_back({c, k0});      // prolonged into from l + 1
       }
       if (coarsest) {
-        if (ly == 1)
-          M.gcx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)].push_back({c, 0, 1, 1});
-        else
-          M.gcy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)].push_back({c, 0, 1, 1});
+        LineGrp g{c, 0, 1, 1};
+        auto &set = ly == 1 ? M.gcx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)]
+                            : M.gcy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)];
+        auto &cyc = ly == 1 ? M.swx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)]
+                            : M.swy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)];
+        set.push_back(g);
+        if (l < lf) cyc.push_back(g);
       } else {
         // groups of tile_lines(shape) lines of one colour
         for (int d = 0; d < 2; ++d) {
@@ -521,6 +561,7 @@ int Batch::rebuild() {
             for (int f = p; f < nl; f += 2 * per) {
               const int cntl = std::min(per, (nl - f + 1) / 2);
               M.gmap[l][d][p][sh].push_back({c, f, cntl, 2});
+              if (l < lf) M.sw[l][d][p][sh].push_back({c, f, cntl, 2});
             }
         }
       }
@@ -537,20 +578,31 @@ int Batch::rebuild() {
           auto len = [&](const LineGrp &g) {
             return d == 0 ? table[g.inst].lev[l].nx : table[g.inst].lev[l].ny;
           };
-          std::stable_sort(v.begin(), v.end(),
-                           [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
-          if (d == 1) xcd_pairs(v);
+          for (auto *w : {&v, &M.sw[l][d][p][sh]}) {
+            std::stable_sort(w->begin(), w->end(),
+                             [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
+            if (d == 1) xcd_pairs(*w);
+          }
         }
+  // fused cycles with the most levels first: they are the critical path
+  std::stable_sort(M.fmap.begin(), M.fmap.end(), [&](const FuseMap &a, const FuseMap &b) {
+    return table[a.inst].nlev - a.lev > table[b.inst].nlev - b.lev;
+  });
   // one device buffer for all maps
   size_t bytes = 0;
   auto sz = [&](size_t n, size_t e) { bytes += a256(n * e); };
+  sz(M.fmap.size(), sizeof(FuseMap));
   for (int l = 0; l < kMaxLev; ++l) {
     sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
+    sz(M.rmap[l].size(), sizeof(BlkMap)); sz(M.qmap[l].size(), sizeof(BlkMap));
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
-        for (int p = 0; p < 2; ++p) sz(M.gmap[l][d][p][sh].size(), sizeof(LineGrp));
-      sz(M.gcx[l][sh].size(), sizeof(LineGrp));
-      sz(M.gcy[l][sh].size(), sizeof(LineGrp));
+        for (int p = 0; p < 2; ++p) {
+          sz(M.gmap[l][d][p][sh].size(), sizeof(LineGrp));
+          sz(M.sw[l][d][p][sh].size(), sizeof(LineGrp));
+        }
+      sz(M.gcx[l][sh].size(), sizeof(LineGrp)); sz(M.gcy[l][sh].size(), sizeof(LineGrp));
+      sz(M.swx[l][sh].size(), sizeof(LineGrp)); sz(M.swy[l][sh].size(), sizeof(LineGrp));
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
@@ -569,33 +621,36 @@ int Batch::rebuild() {
     return p;
   };
   DevMaps dmn{};
+  dmn.fmap = (const FuseMap *)put(M.fmap.data(), M.fmap.size(), sizeof(FuseMap));
+  dmn.nfmap = (int)M.fmap.size();
   for (int l = 0; l < kMaxLev; ++l) {
     dmn.cmap[l] = (const BlkMap *)put(M.cmap[l].data(), M.cmap[l].size(), sizeof(BlkMap));
     dmn.ncmap[l] = (int)M.cmap[l].size();
     dmn.pmap[l] = (const BlkMap *)put(M.pmap[l].data(), M.pmap[l].size(), sizeof(BlkMap));
     dmn.npmap[l] = (int)M.pmap[l].size();
+    dmn.rmap[l] = (const BlkMap *)put(M.rmap[l].data(), M.rmap[l].size(), sizeof(BlkMap));
+    dmn.nrmap[l] = (int)M.rmap[l].size();
+    dmn.qmap[l] = (const BlkMap *)put(M.qmap[l].data(), M.qmap[l].size(), sizeof(BlkMap));
+    dmn.nqmap[l] = (int)M.qmap[l].size();
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
           auto &v = M.gmap[l][d][p][sh];
           dmn.gmap[l][d][p][sh] = (const LineGrp *)put(v.data(), v.size(), sizeof(LineGrp));
           dmn.ngmap[l][d][p][sh] = (int)v.size();
+          auto &w = M.sw[l][d][p][sh];
+          dmn.sw[l][d][p][sh] = (const LineGrp *)put(w.data(), w.size(), sizeof(LineGrp));
+          dmn.nsw[l][d][p][sh] = (int)w.size();
         }
       dmn.gcx[l][sh] = (const LineGrp *)put(M.gcx[l][sh].data(), M.gcx[l][sh].size(), sizeof(LineGrp));
       dmn.ngcx[l][sh] = (int)M.gcx[l][sh].size();
       dmn.gcy[l][sh] = (const LineGrp *)put(M.gcy[l][sh].data(), M.gcy[l][sh].size(), sizeof(LineGrp));
       dmn.ngcy[l][sh] = (int)M.gcy[l][sh].size();
+      dmn.swx[l][sh] = (const LineGrp *)put(M.swx[l][sh].data(), M.swx[l][sh].size(), sizeof(LineGrp));
+      dmn.nswx[l][sh] = (int)M.swx[l][sh].size();
+      dmn.swy[l][sh] = (const LineGrp *)put(M.swy[l][sh].data(), M.swy[l][sh].size(), sizeof(LineGrp));
+      dmn.nswy[l][sh] = (int)M.swy[l][sh].size();
     }
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p) {
-        double c = 0;
-        for (int sh = 0; sh < kNumTileShapes; ++sh)
-          for (const LineGrp &g : M.gmap[l][d][p][sh]) {
-            const InstDev &I = table[g.inst];
-            c += (double)g.count * (d == 0 ? I.lev[l].nx : I.lev[l].ny);
-          }
-        dmn.gcells[l][d][p] = c;
-      }
   }
   dmn.maxlev = maxlev;
   if (bytes) HIPCHK(hipMemcpy(d_maps, host.data(), bytes, hipMemcpyHostToDevice));
@@ -653,35 +708,42 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
       for (int sh = 0; sh < kNumTileShapes; ++sh) {
-        if (!m.ngmap[l][dir][p][sh]) continue;
+        if (!m.nsw[l][dir][p][sh]) continue;
         tbegin(lid);   // one timed interval per kernel dispatch
-        launch_line<T>(stream, d_insts, m.gmap[l][dir][p][sh], m.ngmap[l][dir][p][sh], l, dir, sh,
+        launch_line<T>(stream, d_insts, m.sw[l][dir][p][sh], m.nsw[l][dir][p][sh], l, dir, sh,
                        zero, xid, bid);
         tend(lid);
       }
       zero = false;
     }
   };
+  // Level l + 1 has per-level work for some instance (`deeper`), or this is
+  // the bottom of the descent. With post-smoothing only, the descent is
+  // restrictions of b, so at the bottom every fused cycle has the b of its
+  // first level and no per-level prolongation has started: all of them run
+  // here, in one launch, one workgroup per instance.
+  const bool deeper = l + 1 < m.maxlev && m.nrmap[l + 1] > 0;
+  if (!deeper) launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm);
   if (pre_x) sweep(0);
   if (pre_y) sweep(1);
   const bool pre_zero = zero;
   // exact solve on the coarsest level of the instances that end here
   tbegin(3);
   for (int sh = 0; sh < kNumTileShapes; ++sh) {
-    launch_line<T>(stream, d_insts, m.gcx[l][sh], m.ngcx[l][sh], l, 0, sh, true, xid, bid);
-    launch_line<T>(stream, d_insts, m.gcy[l][sh], m.ngcy[l][sh], l, 1, sh, true, xid, bid);
+    launch_line<T>(stream, d_insts, m.swx[l][sh], m.nswx[l][sh], l, 0, sh, true, xid, bid);
+    launch_line<T>(stream, d_insts, m.swy[l][sh], m.nswy[l][sh], l, 1, sh, true, xid, bid);
   }
   tend(3);
-  if (l + 1 < m.maxlev && m.ncmap[l + 1] > 0) {
+  if (deeper) {
     tbegin(4);
     if (!pre_zero) launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, xid, bid);
-    launch_restrict<T>(stream, d_insts, m.cmap[l + 1], m.ncmap[l + 1], l, pre_zero, xid, bid);
+    launch_restrict<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, pre_zero, xid, bid);
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
     // only the lines the first post-smoothing sweep reads are needed
     const int rows = post_x ? 1 : post_y ? 2 : 0;
-    launch_prolong<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, pre_zero, xid, rows);
+    launch_prolong<T>(stream, d_insts, m.qmap[l], m.nqmap[l], l, pre_zero, xid, rows);
     tend(5);
     zero = false;
     if (post_x) sweep(0);
@@ -835,6 +897,7 @@ int Batch::step_once() {
     for (int c = 0; c < n; ++c) {
       if (!part[c]) continue;
       const Slot &s = slots[order[c]];
+      if (s.fuse_lev == 0) continue;   // its level 0 runs in the fused coarse cycle
       t_bytes += h_scal[order[c]].vcyc * 2.0 * 8.0 * vt * (double)s.nx * s.ny;
     }
   }

@@ -119,5 +119,12 @@ struct InstDev {
 
 struct BlkMap { int inst; int k0; };      // cell-parallel: 256 cells of one instance
 struct LineGrp { int inst; int first; int count; int step; };   // lines first + k*step, k < count
+struct FuseMap { int inst; int lev; };    // an instance whose levels from lev down run fused
+
+// The coarse end of a V-cycle runs in one workgroup per instance from the
+// first level with at most fuse_cells cells (the fused coarse cycle; default
+// kFuseCells).
+constexpr int kFuseCells = 1024;
+constexpr int kFuseThreads = 1024;
 
 }  // namespace sg

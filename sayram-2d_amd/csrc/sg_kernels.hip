@@ -932,16 +932,9 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
 // contiguous across the wave; the coarse store alternates between the two
 // contiguous halves of the coarse row.
 template <typename T, bool FROM_B>
-__global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
-                                                    const BlkMap *__restrict__ map, int l,
-                                                    int xid, int bid) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+__device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid, int bid, int KC) {
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
-  const int KC = m.k0 + threadIdx.x;
-  if (KC >= Cl.nx * Cl.ny) return;
   const int J = KC / Cl.nx, Ic = KC - J * Cl.nx;    // column order
   const int hxf = half_x(F.nx);
   T *x; const T *b;
@@ -968,6 +961,18 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
 }
 
+template <typename T, bool FROM_B>
+__global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
+                                                    const BlkMap *__restrict__ map, int l,
+                                                    int xid, int bid) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (I.sc->status != ST_ACTIVE) return;
+  const int KC = m.k0 + threadIdx.x;
+  if (KC >= I.lev[l + 1].nx * I.lev[l + 1].ny) return;
+  restrict_point<T, FROM_B>(I, l, xid, bid, KC);
+}
+
 // x_l += P x_{l+1}. ZERO: x_l = P x_{l+1}.
 // ROWS 0: every cell. ROWS 1: odd rows only; ROWS 2: odd columns only. A
 // zebra line sweep solves its colour-0 lines from their neighbours and never
@@ -975,16 +980,14 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
 // after the prolongation only the colour-1 lines of the first post-smoothing
 // sweep are read (rows 1, 3, .. for x-lines; columns 1, 3, .. for y-lines).
 // The block map covers the level; blocks past the compact range exit.
+template <int ROWS> __device__ __forceinline__ int prolong_count(const Lev &F) {
+  return ROWS == 1 ? F.nx * (F.ny >> 1) : ROWS == 2 ? (F.nx - half_x(F.nx)) * F.ny : F.nx * F.ny;
+}
+
 template <typename T, bool ZERO, int ROWS>
-__global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ insts,
-                                                   const BlkMap *__restrict__ map, int l,
-                                                   int xid) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+__device__ __forceinline__ void prolong_point(const InstDev &I, int l, int xid, int idx) {
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
-  const int idx = m.k0 + threadIdx.x;
   const int hx = half_x(F.nx);
   int K, i, j;
   if (ROWS == 1) {            // rows 1, 3, ..: nx cells each
@@ -1014,6 +1017,191 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
   const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x));
   x[K] = ZERO ? e : x[K] + e;
 }
+
+template <typename T, bool ZERO, int ROWS>
+__global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ insts,
+                                                   const BlkMap *__restrict__ map, int l,
+                                                   int xid) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (I.sc->status != ST_ACTIVE) return;
+  const int idx = m.k0 + threadIdx.x;
+  if (idx >= prolong_count<ROWS>(I.lev[l])) return;
+  prolong_point<T, ZERO, ROWS>(I, l, xid, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Fused coarse cycle: the levels of one instance from `lev` (its first level
+// with at most kFuseCells cells) down to its coarsest and back, in one
+// workgroup with barriers between the phases, instead of several launches per
+// level. The cycle is the per-level one with post-smoothing only: restriction
+// of b to the coarsest level, its exact line solve, then per level the
+// prolongation of the lines the first sweep reads and the zebra sweeps.
+//
+// A level being smoothed lives in LDS (x, and per sweep the line right-hand
+// sides and factors), so only the parallel phases touch global memory: every
+// thread forms the right-hand sides of some cells (b minus the couplings to
+// the other colour's lines) and copies their factors, then one thread per line
+// runs the two Thomas recurrences out of LDS (lines are short here). The line
+// solves are the per-level ones rounded in another order.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FuseLds {
+  T x[kFuseCells];     // x of the level being smoothed
+  T r[kFuseCells];     // line right-hand sides, then d of the forward pass
+  T lo[kFuseCells];    // coupling to the previous cell of the line (0 at t = 0)
+  T inv[kFuseCells];   // 1 / pivot
+  T up[kFuseCells];    // upper / pivot
+};
+
+template <typename T, int NB, int DIR>
+__device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool zero, FuseLds<T> &S) {
+  const Lev &Lv = I.lev[l];
+  const int nx = Lv.nx, ny = Lv.ny, hx = half_x(nx);
+  const T *b = l == 0 ? cptr<T>(vec(I, bid)) : cptr<T>(Lv.b);
+  const T *lo = cptr<T>(Lv.band[DIR == 0 ? BW : BS]);
+  const T *om = cptr<T>(Lv.band[DIR == 0 ? BS : BW]);
+  const T *op = cptr<T>(Lv.band[DIR == 0 ? BN : BE]);
+  const T *omm = nullptr, *omp = nullptr, *opm = nullptr, *opp = nullptr;
+  if (NB == 9) {
+    omm = cptr<T>(Lv.band[BSW]);
+    omp = cptr<T>(Lv.band[DIR == 0 ? BSE : BNW]);
+    opm = cptr<T>(Lv.band[DIR == 0 ? BNW : BSE]);
+    opp = cptr<T>(Lv.band[BNE]);
+  }
+  const T *inv = cptr<T>(Lv.linv[DIR]);
+  const T *cu = cptr<T>(Lv.lup[DIR]);
+  const int n = DIR == 0 ? nx : ny, nperp = DIR == 0 ? ny : nx;
+  const int nl = (nperp - colour + 1) >> 1;   // lines colour, colour + 2, ..
+  auto at = [&](int li, int t) -> uint32_t {
+    return DIR == 0 ? (uint32_t)(li * nx + xslot(t, hx)) : (uint32_t)(t * nx + xslot(li, hx));
+  };
+  for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
+    const int q = idx / n, t = idx - q * n, li = colour + 2 * q;
+    const uint32_t K = at(li, t);
+    T r = ld32(b, K);
+    if (!zero) {
+      if (li > 0) {
+        r -= ld32(om, K) * S.x[at(li - 1, t)];
+        if (NB == 9) {
+          if (t > 0) r -= ld32(omm, K) * S.x[at(li - 1, t - 1)];
+          if (t < n - 1) r -= ld32(omp, K) * S.x[at(li - 1, t + 1)];
+        }
+      }
+      if (li < nperp - 1) {
+        r -= ld32(op, K) * S.x[at(li + 1, t)];
+        if (NB == 9) {
+          if (t > 0) r -= ld32(opm, K) * S.x[at(li + 1, t - 1)];
+          if (t < n - 1) r -= ld32(opp, K) * S.x[at(li + 1, t + 1)];
+        }
+      }
+    }
+    S.r[K] = r;
+    S.lo[K] = t > 0 ? ld32(lo, K) : T(0);
+    S.inv[K] = ld32(inv, K);
+    S.up[K] = ld32(cu, K);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+    const int li = colour + 2 * q;
+    T d = T(0);
+    for (int t = 0; t < n; ++t) {   // forward: d_t = (r_t - lo_t d_{t-1}) / pivot_t
+      const uint32_t K = at(li, t);
+      d = (S.r[K] - S.lo[K] * d) * S.inv[K];
+      S.r[K] = d;
+    }
+    T xn = T(0);
+    for (int t = n - 1; t >= 0; --t) {   // backward: x_t = d_t - u_t x_{t+1}
+      const uint32_t K = at(li, t);
+      xn = S.r[K] - S.up[K] * xn;
+      S.x[K] = xn;
+    }
+  }
+  __syncthreads();
+}
+
+// x of level l: global -> LDS (load) or LDS -> global (store); ends with a
+// barrier.
+template <typename T>
+__device__ void fused_x(const InstDev &I, int l, int xid, FuseLds<T> &S, bool load) {
+  const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(I.lev[l].x));
+  const int nc = I.lev[l].nx * I.lev[l].ny;
+  for (int K = threadIdx.x; K < nc; K += blockDim.x) {
+    if (load) S.x[K] = x[K];
+    else x[K] = S.x[K];
+  }
+  __syncthreads();
+}
+
+template <typename T, int NB>
+__device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool post_x, bool post_y,
+                             FuseLds<T> &S) {
+  fused_x<T>(I, l, xid, S, true);
+  if (post_x) {
+    fused_sweep<T, NB, 0>(I, l, bid, 0, false, S);
+    fused_sweep<T, NB, 0>(I, l, bid, 1, false, S);
+  }
+  if (post_y) {
+    fused_sweep<T, NB, 1>(I, l, bid, 0, false, S);
+    fused_sweep<T, NB, 1>(I, l, bid, 1, false, S);
+  }
+  fused_x<T>(I, l, xid, S, false);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__restrict__ insts,
+                                                               const FuseMap *__restrict__ map,
+                                                               int xid, int bid, int smoother) {
+  __shared__ FuseLds<T> S;
+  const FuseMap fm = map[blockIdx.x];
+  const InstDev &I = insts[fm.inst];
+  if (I.sc->status != ST_ACTIVE) return;
+  const int lf = fm.lev, nl = I.nlev;
+  const bool post_x = smoother & 4, post_y = smoother & 8;
+  const int rows = post_x ? 1 : post_y ? 2 : 0;
+  for (int m = lf; m + 1 < nl; ++m) {   // b_{m+1} = R b_m (x_m = 0)
+    const int nc = I.lev[m + 1].nx * I.lev[m + 1].ny;
+    for (int KC = threadIdx.x; KC < nc; KC += blockDim.x) restrict_point<T, true>(I, m, xid, bid, KC);
+    __syncthreads();
+  }
+  {   // exact solve of the coarsest level (x = 0): one row or one column
+    const int c = nl - 1;
+    if (I.lev[c].ny == 1) {
+      if (c == 0) fused_sweep<T, 5, 0>(I, c, bid, 0, true, S);
+      else fused_sweep<T, 9, 0>(I, c, bid, 0, true, S);
+    } else {
+      if (c == 0) fused_sweep<T, 5, 1>(I, c, bid, 0, true, S);
+      else fused_sweep<T, 9, 1>(I, c, bid, 0, true, S);
+    }
+    fused_x<T>(I, c, xid, S, false);
+  }
+  for (int m = nl - 2; m >= lf; --m) {  // x_m = P x_{m+1}, then post-smoothing
+    const Lev &F = I.lev[m];
+    if (rows == 1) {
+      const int np = prolong_count<1>(F);
+      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 1>(I, m, xid, k);
+    } else if (rows == 2) {
+      const int np = prolong_count<2>(F);
+      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 2>(I, m, xid, k);
+    } else {
+      const int np = prolong_count<0>(F);
+      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 0>(I, m, xid, k);
+    }
+    __syncthreads();
+    if (m == 0) fused_smooth<T, 5>(I, m, xid, bid, post_x, post_y, S);
+    else fused_smooth<T, 9>(I, m, xid, bid, post_x, post_y, S);
+  }
+}
+
+template <typename T>
+void launch_coarse_cycle(hipStream_t s, const InstDev *d, const FuseMap *m, int n, int xid, int bid,
+                         int smoother) {
+  if (n) k_coarse_cycle<T><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+}
+template void launch_coarse_cycle<double>(hipStream_t, const InstDev *, const FuseMap *, int, int,
+                                          int, int);
+template void launch_coarse_cycle<float>(hipStream_t, const InstDev *, const FuseMap *, int, int,
+                                         int, int);
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------

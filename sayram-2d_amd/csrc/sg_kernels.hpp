@@ -28,6 +28,11 @@ void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, i
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid, int rows);
+// the fused coarse cycle of the instances in m (one workgroup each); post-
+// smoothing schedules only (smoother bits 4 and 8)
+template <typename T>
+void launch_coarse_cycle(hipStream_t s, const InstDev *d, const FuseMap *m, int n, int xid, int bid,
+                         int smoother);
 void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 // which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update.
 // pc: FP64 Krylov vectors with FP32 preconditioned vectors (T must be double).

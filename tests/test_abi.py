@@ -56,6 +56,16 @@ def test_no_gpu_fails_loudly():
     assert e.value.code == sg.SG_ERR_HIP
 
 
+def test_option_errors_come_before_any_device_call():
+    for opts in (sg.make_opts(precision=3), sg.make_opts(fuse_cells=1025)):
+        with pytest.raises(sg.SayramError) as e:
+            sg.Batch(opts)
+        assert e.value.code == sg.SG_ERR_ARG
+    with pytest.raises(sg.SayramError) as e:
+        sg.Batch(sg.make_opts(fuse_cells=2048))
+    assert "fuse_cells" in str(e.value)
+
+
 def test_instance_bytes_accounting():
     b64 = [sg.Batch.instance_bytes(n, n, sg.SG_FP64) for n in (32, 64, 512, 4096)]
     assert all(x < y for x, y in zip(b64, b64[1:]))

@@ -116,3 +116,27 @@ def test_graph_replay_equals_eager(gpu_lib):
                 assert np.array_equal(bg.f(a), be.f(b)), f"precision {prec}, step {step}"
         bg.close()
         be.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
+    """The fused coarse cycle (one launch per instance from its first level of
+    at most fuse_cells cells) solves the same coarse problems as the per-level
+    launches: trajectories agree to rounding, and both stay on the oracle.
+    Shapes cover fusion from level 0 (small grids), from a coarse level, and
+    a batch mixing the two."""
+    probs = [sc.seeded_case(nx, ny, seed=90 + k, scale=s) for k, (nx, ny, s) in
+             enumerate([(24, 20, 2.0), (70, 45, 0.5), (33, 130, 5.0)])]
+    out = {}
+    for fc in (0, -1, 300):
+        b = gpu_lib.Batch(gpu_lib.make_opts(precision=precision, fuse_cells=fc))
+        slots = [b.add(p) for p in probs]
+        b.step(4)
+        out[fc] = [b.f(s) for s in slots]
+        b.close()
+    for k, p in enumerate(probs):
+        ref = orc.run(orc.OracleProblem(p), p.f0, 4)
+        for fc in (0, 300):
+            assert rel_l2(out[fc][k], out[-1][k]) <= 1e-11
+        for fc in out:
+            assert rel_l2(out[fc][k], ref) <= TOL
