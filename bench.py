@@ -79,9 +79,15 @@ def cpu_model():
 
 
 def run_batched(probs, warmup, steps, precision=sg.SG_MIXED, graph=True, time_kernel=True,
-                opts=None):
-    b = sg.Batch(opts if opts is not None else
-                 sg.make_opts(precision=precision, use_graph=1 if graph else -1))
+                opts=None, streams=0):
+    """Batched steps. time_kernel brackets every level-0 sweep dispatch with
+    HIP events; such a run launches eagerly and on one stream (streams=1), so
+    that each dispatch's interval is that kernel alone."""
+    if opts is None:
+        opts = sg.make_opts(precision=precision, use_graph=1 if graph else -1, streams=streams)
+    if time_kernel:
+        opts.use_graph, opts.streams = -1, 1
+    b = sg.Batch(opts)
     slots = [b.add(p) for p in probs]
     b.step(warmup)
     if time_kernel:
@@ -182,6 +188,8 @@ def main():
                     help="precision of the batched step that is measured")
     ap.add_argument("--graph", choices=["on", "off"], default="on",
                     help="replay the Krylov chunks from captured hipGraphs")
+    ap.add_argument("--streams", type=int, choices=[0, 1, 2], default=0,
+                    help="HIP streams of the line sweeps' tile shapes (0: library default, 2)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch parameters on the workload first (sayram_tune) and "
                          "time the step with the best set")
@@ -192,7 +200,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64,
                     help="requests in the serving trace (0 skips the serving measurement)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="only the timed batched steps (for rocprofv3)")
+                    help="only the timed batched steps (for rocprofv3); with --graph off, only "
+                         "the kernel-timing run (eager, one stream, HIP events per dispatch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times: the known-size kernel that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
@@ -229,13 +238,32 @@ def main():
         best, bv, trials = tu.tune(probs, prec, steps=1, warmup=1, log=log)
         tuned = {"best": best, "value": bv, "trials": len(trials)}
         graph = best["use_graph"] == 1
-    opts = None
-    if tuned:
-        p = dict(tuned["best"], use_graph=1 if graph else -1)
-        opts = tu.opts_for(prec, p)
-    # HIP-event kernel timing records host-side events, so it runs eagerly
-    dt, (kms, kl, kbytes), st = run_batched(probs, args.warmup, args.steps, prec, graph=graph,
-                                            time_kernel=not graph, opts=opts)
+    def tuned_opts(use_graph):
+        if not tuned:
+            return None
+        o = tu.opts_for(prec, dict(tuned["best"], use_graph=use_graph))
+        o.streams = args.streams
+        return o
+
+    def kernel_timing_run():
+        # HIP-event timing of the dominant kernel records events from the host
+        # around every dispatch: eager launches on one stream (same kernels,
+        # same launch sizes as the headline run)
+        dte, kt, _ = run_batched(probs, args.warmup, args.steps, prec, graph=False,
+                                 time_kernel=True, opts=tuned_opts(-1))
+        log(f"[bench] eager launches, one stream (kernel timing): {dte / args.steps * 1e3:.2f} ms/step")
+        if kt[1] and kt[0] > 0:
+            kms, kl, kbytes = kt
+            log(f"[bench] {KERNEL_DESC}: {kl} dispatches, HIP-event mean {kms / kl * 1e3:.1f} us, "
+                f"{kbytes / kl / 1e6:.1f} MB algorithmic per dispatch, "
+                f"{kbytes / (kms * 1e-3) / 1e9:.0f} GB/s")
+        return dte, kt
+
+    if args.profile_only and not graph:
+        kernel_timing_run()
+        return
+    dt, _, st = run_batched(probs, args.warmup, args.steps, prec, graph=graph, time_kernel=False,
+                            opts=tuned_opts(1 if graph else -1), streams=args.streams)
     value = cells * args.steps / dt
     iters = [s["last_iters"] for s in st]
     refs = [s["last_refinements"] for s in st]
@@ -243,10 +271,6 @@ def main():
         f"cell-steps/s, Krylov iterations mean {np.mean(iters):.1f} max {max(iters)}, "
         f"refinements mean {np.mean(refs):.2f}")
     if args.profile_only:
-        if kl and kms > 0:
-            log(f"[bench] {KERNEL_DESC}: {kl} dispatches, HIP-event mean {kms / kl * 1e3:.1f} us, "
-                f"{kbytes / kl / 1e6:.1f} MB algorithmic per dispatch, "
-                f"{kbytes / (kms * 1e-3) / 1e9:.0f} GB/s")
         return
 
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
@@ -259,21 +283,14 @@ def main():
                                   f"{args.big}^2 grid, dt = 1e-3 day",
                       "precision": args.precision,
                       "instances": len(probs), "cells": cells, "seed": args.seed},
-           "launch": {"graph": graph, "autotuned": tuned},
+           "launch": {"graph": graph, "streams": args.streams or 2, "autotuned": tuned},
            "solver": {"inner_iterations_mean": float(np.mean(iters)), "inner_iterations_max": int(max(iters)),
                       "refinements_mean": float(np.mean(refs)),
                       "min_f": float(min(s["min_f"] for s in st)),
                       "clamped_cells": int(sum(s["clamp_count"] for s in st)),
                       "max_rel_residual": float(max(s["last_rel_residual"] for s in st))}}
-    if graph:
-        # HIP-event timing of the dominant kernel is recorded from the host, so
-        # it runs over an equal timed region of eager launches (same kernels,
-        # same launch sizes); its step time is reported beside the graph run's
-        eopts = None if opts is None else tu.opts_for(prec, dict(tuned["best"], use_graph=-1))
-        dte, (kms, kl, kbytes), _ = run_batched(probs, args.warmup, args.steps, prec, graph=False,
-                                                time_kernel=True, opts=eopts)
-        out["launch"]["eager_ms_per_step"] = dte / args.steps * 1e3
-        log(f"[bench] eager launches (kernel timing): {dte / args.steps * 1e3:.2f} ms/step")
+    dte, (kms, kl, kbytes) = kernel_timing_run()
+    out["launch"]["eager_one_stream_ms_per_step"] = dte / args.steps * 1e3
     if kl and kms > 0:
         avg_ms = kms / kl
         achieved = kbytes / (kms * 1e-3) / 1e9
@@ -281,6 +298,9 @@ def main():
                            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_launch_ms": avg_ms, "launches": kl,
+                           "timing": "HIP events around every dispatch on its stream, in an "
+                                     "eager run on one stream so that each interval is the "
+                                     "kernel alone (the headline run overlaps tile shapes)",
                            "algorithmic_bytes_per_launch": kbytes / kl}
         # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
         # runs recorded beside the kernel-trace summary

@@ -102,7 +102,9 @@ typedef struct {
                            0 selects 1024, the maximum (a level is kept in
                            LDS); -1 disables. Like line_tiles it changes the
                            rounding of those line solves. */
-  int reserved[1];
+  int streams;          /* 2: the tile shapes of one line sweep run on two HIP
+                           streams (the longest lines beside the rest); 1: one
+                           stream. 0 selects 2. Results are identical. */
 } sg_opts;
 
 typedef struct {

@@ -198,6 +198,12 @@ struct Batch {
   sg_opts opts{};
   size_t budget = 0, in_use = 0;
   hipStream_t stream = nullptr;
+  // Second stream (opts.streams == 2): the tile shapes of one sweep solve
+  // disjoint lines, so the shape with the longest lines runs on `side` beside
+  // the others on `stream`; `stream` waits for it before the next sweep.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipError_t fan_err = hipSuccess;   // first failure of the fork/join calls
   std::vector<Slot> slots;
   int capacity = 0;
   InstDev *d_insts = nullptr;   // compact table
@@ -247,14 +253,15 @@ struct Batch {
   template <typename KT, typename VT> int krylov(int maxit, bool xstate);
   template <typename KT, typename VT> void iteration(bool xstate);
   template <typename T> void vcycle(int l, int xid, int bid);
+  template <typename F> void fan_out(const int (&cnt)[kNumTileShapes], F &&launch);
   int read_counter(int which, int *out);
   int scratch(size_t n);
   int get_f(int slot, double *out);
   int set_f(int slot, const double *f, int64_t istep);
   int debug_assemble(int slot, double *bands, double *R);
   // timing helpers
-  void tbegin(int id);
-  void tend(int id);
+  void tbegin(int id, hipStream_t s = nullptr);
+  void tend(int id, hipStream_t s = nullptr);
   int collect_time();
 };
 
@@ -270,6 +277,9 @@ Batch::~Batch() {
   for (auto ev : ev_cnt)
     if (ev) (void)hipEventDestroy(ev);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -281,6 +291,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.check_every <= 0) o.check_every = 1;
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
+  if (o.streams == 0) o.streams = 2;
   const int tiles[4] = {24, 160, 24, 384};
   for (int k = 0; k < 4; ++k)
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
@@ -293,11 +304,17 @@ int Batch::init(const sg_opts *o, size_t b) {
   apply_defaults(opts);
   if (opts.fuse_cells > kFuseCells)
     return fail(SG_ERR_ARG, "opts.fuse_cells must be at most 1024 (the fused cycle keeps a level in LDS)");
+  if (opts.streams != 1 && opts.streams != 2) return fail(SG_ERR_ARG, "opts.streams must be 0, 1 or 2");
   budget = b;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(SG_ERR_HIP, "no GPU visible: the PPFV step has no CPU fallback");
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (opts.streams == 2) {
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  }
   HIPCHK(hipMalloc(&d_cnt, 2 * sizeof(int)));
   HIPCHK(hipHostMalloc((void **)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&h_pipe, 2 * sizeof(int), hipHostMallocDefault));
@@ -522,6 +539,7 @@ int Batch::rebuild() {
   Maps M;
   order.clear();
   const bool f64 = opts.precision == SG_FP64;   // FP64 V-cycle
+  const int prow = prolong_rows(opts.smoother);
   std::vector<InstDev> table;
   int maxlev = 0;
   for (size_t k = 0; k < slots.size(); ++k) {
@@ -541,8 +559,11 @@ int Batch::rebuild() {
         M.cmap[l].push_back({c, k0});
         if (!coarsest) M.pmap[l].push_back({c, k0});
         if (l >= 1 && l <= lf) M.rmap[l].push_back({c, k0});        // restricted into from l - 1
-        if (!coarsest && l < lf) M.qmap[l].push_back({c, k0});      // prolonged into from l + 1
       }
+      // prolonged into from l + 1: only the lines the first post-sweep reads
+      const int nq = prow == 1 ? lx * (ly >> 1) : prow == 2 ? (lx - half_x(lx)) * ly : Nl;
+      if (!coarsest && l < lf)
+        for (int k0 = 0; k0 < nq; k0 += kBlock) M.qmap[l].push_back({c, k0});
       if (coarsest) {
         LineGrp g{c, 0, 1, 1};
         auto &set = ly == 1 ? M.gcx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)]
@@ -669,20 +690,21 @@ int Batch::read_counter(int which, int *out) {
   return SG_OK;
 }
 
-void Batch::tbegin(int id) {
+void Batch::tbegin(int id, hipStream_t s) {
   if (id != time_id) return;
+  if (!s) s = stream;
   if (ev_used + 2 > ev_pool.size()) {
     for (int k = 0; k < 64; ++k) {
       hipEvent_t e;
       if (hipEventCreate(&e) == hipSuccess) ev_pool.push_back(e);
     }
   }
-  if (ev_used + 2 <= ev_pool.size()) (void)hipEventRecord(ev_pool[ev_used], stream);
+  if (ev_used + 2 <= ev_pool.size()) (void)hipEventRecord(ev_pool[ev_used], s);
 }
 
-void Batch::tend(int id) {
+void Batch::tend(int id, hipStream_t s) {
   if (id != time_id || ev_used + 2 > ev_pool.size()) return;
-  (void)hipEventRecord(ev_pool[ev_used + 1], stream);
+  (void)hipEventRecord(ev_pool[ev_used + 1], s ? s : stream);
   ev_used += 2;
 }
 
@@ -699,6 +721,30 @@ int Batch::collect_time() {
   return SG_OK;
 }
 
+// One launch per tile shape with work (launch(stream, shape)). With a side
+// stream and two or more shapes, the last shape (the longest lines: its groups
+// are the critical path) runs on `side`, forked from and joined back into
+// `stream`; the shapes write disjoint lines, so the results do not change.
+// Under stream capture the fork and join become graph edges.
+template <typename F> void Batch::fan_out(const int (&cnt)[kNumTileShapes], F &&launch) {
+  int nshape = 0, last = -1;
+  for (int sh = 0; sh < kNumTileShapes; ++sh)
+    if (cnt[sh]) { ++nshape; last = sh; }
+  if (nshape < 2 || !side) {
+    for (int sh = 0; sh < kNumTileShapes; ++sh)
+      if (cnt[sh]) launch(stream, sh);
+    return;
+  }
+  auto note = [&](hipError_t e) { if (e != hipSuccess && fan_err == hipSuccess) fan_err = e; };
+  note(hipEventRecord(ev_fork, stream));
+  note(hipStreamWaitEvent(side, ev_fork, 0));
+  launch(side, last);
+  note(hipEventRecord(ev_join, side));
+  for (int sh = 0; sh < last; ++sh)
+    if (cnt[sh]) launch(stream, sh);
+  note(hipStreamWaitEvent(stream, ev_join, 0));
+}
+
 template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const DevMaps &m = dm;
   const int sm = opts.smoother;
@@ -707,13 +753,12 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      for (int sh = 0; sh < kNumTileShapes; ++sh) {
-        if (!m.nsw[l][dir][p][sh]) continue;
-        tbegin(lid);   // one timed interval per kernel dispatch
-        launch_line<T>(stream, d_insts, m.sw[l][dir][p][sh], m.nsw[l][dir][p][sh], l, dir, sh,
-                       zero, xid, bid);
-        tend(lid);
-      }
+      fan_out(m.nsw[l][dir][p], [&](hipStream_t s, int sh) {
+        tbegin(lid, s);   // one timed interval per kernel dispatch
+        launch_line<T>(s, d_insts, m.sw[l][dir][p][sh], m.nsw[l][dir][p][sh], l, dir, sh, zero,
+                       xid, bid);
+        tend(lid, s);
+      });
       zero = false;
     }
   };
@@ -742,8 +787,7 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
     // only the lines the first post-smoothing sweep reads are needed
-    const int rows = post_x ? 1 : post_y ? 2 : 0;
-    launch_prolong<T>(stream, d_insts, m.qmap[l], m.nqmap[l], l, pre_zero, xid, rows);
+    launch_prolong<T>(stream, d_insts, m.qmap[l], m.nqmap[l], l, pre_zero, xid, prolong_rows(sm));
     tend(5);
     zero = false;
     if (post_x) sweep(0);
@@ -800,6 +844,7 @@ template <typename KT, typename VT> int Batch::krylov(int maxit, bool xstate) {
       for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
     }
     it += c;
+    HIPCHK(fan_err);
     HIPCHK(hipMemcpyAsync(h_pipe + slot, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cnt[slot], stream));
     return SG_OK;
@@ -845,11 +890,12 @@ int Batch::step_once() {
     tbegin(1);
     // LU factors of the line matrices: smoothing lines of both colours and the
     // exact solve on the coarsest level
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p)
+        fan_out(dm.ngmap[l][d][p], [&](hipStream_t s, int sh) {
+          launch_line_factor(s, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d, sh, h32);
+        });
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      for (int d = 0; d < 2; ++d)
-        for (int p = 0; p < 2; ++p)
-          launch_line_factor(stream, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d,
-                             sh, h32);
       launch_line_factor(stream, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
       launch_line_factor(stream, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
     }

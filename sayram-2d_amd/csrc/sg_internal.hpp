@@ -42,6 +42,13 @@ __host__ __device__ constexpr int tile_lines(int sh) {
 __host__ __device__ constexpr int tile_threads(int sh) { return sh == 3 ? 1024 : 256; }
 
 __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
+
+// The prolongation fills only the lines the first post-smoothing sweep reads:
+// odd rows before x-lines (1), odd columns before y-lines (2), else every
+// point (0). `smoother` bits: 1/2 pre x/y, 4/8 post x/y.
+__host__ __device__ inline int prolong_rows(int smoother) {
+  return (smoother & 4) ? 1 : (smoother & 8) ? 2 : 0;
+}
 // position of column i within its row
 __host__ __device__ inline int xslot(int i, int hx) { return (i & 1) ? hx + (i >> 1) : (i >> 1); }
 // column stored at position p of a row
@@ -125,6 +132,8 @@ struct FuseMap { int inst; int lev; };    // an instance whose levels from lev d
 // first level with at most fuse_cells cells (the fused coarse cycle; default
 // kFuseCells).
 constexpr int kFuseCells = 1024;
+// Threads of the per-instance finish kernels (sums of the partials).
+constexpr int kFinThreads = 1024;
 constexpr int kFuseThreads = 1024;
 
 }  // namespace sg

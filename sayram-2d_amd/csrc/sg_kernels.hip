@@ -75,11 +75,12 @@ template <typename T> __device__ __forceinline__ const T *cptr(const void *p) {
 }
 
 // ---------------------------------------------------------------------------
-// workgroup reduction: wave64 shuffles, then LDS across the 4 waves
+// workgroup reduction of NT threads: wave64 shuffles, then LDS across the
+// NT / 64 waves
 // ---------------------------------------------------------------------------
-template <typename A, int NV>
+template <typename A, int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(A (&v)[NV]) {
-  __shared__ A red[NV][kBlock / 64];
+  __shared__ A red[NV][NT / 64];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     A s = v[k];
@@ -97,7 +98,7 @@ __device__ __forceinline__ void block_sum(A (&v)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       A s = red[k][0];
-      for (int w = 1; w < kBlock / 64; ++w) s += red[k][w];
+      for (int w = 1; w < NT / 64; ++w) s += red[k][w];
       v[k] = s;
     }
   }
@@ -979,7 +980,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
 // reads the lines it solves, and the colour-1 lines it reads are solved next:
 // after the prolongation only the colour-1 lines of the first post-smoothing
 // sweep are read (rows 1, 3, .. for x-lines; columns 1, 3, .. for y-lines).
-// The block map covers the level; blocks past the compact range exit.
+// The block map covers that compact range.
 template <int ROWS> __device__ __forceinline__ int prolong_count(const Lev &F) {
   return ROWS == 1 ? F.nx * (F.ny >> 1) : ROWS == 2 ? (F.nx - half_x(F.nx)) * F.ny : F.nx * F.ny;
 }
@@ -1011,9 +1012,16 @@ __device__ __forceinline__ void prolong_point(const InstDev &I, int l, int xid, 
   const int hxc = half_x(Cl.nx);
   const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
   const int s0 = xslot(I0, hxc), s1 = xslot(I1, hxc);
-  // weights of parents outside the coarse grid are zero
-  const T e = gbl(cptr<T>(F.pw[0]))[K] * xc[r0 + s0] + gbl(cptr<T>(F.pw[1]))[K] * xc[r0 + s1] +
-              gbl(cptr<T>(F.pw[2]))[K] * xc[r1 + s0] + gbl(cptr<T>(F.pw[3]))[K] * xc[r1 + s1];
+  // Weights of parents outside the coarse grid are zero, and so are the
+  // weights to the parents at I0 + 1 of an even i and at J0 + 1 of an even j
+  // (k_pweights writes exact zeros there): those terms are skipped, so half
+  // of each row's weight runs is not read.
+  T e = gbl(cptr<T>(F.pw[0]))[K] * xc[r0 + s0];
+  if (i & 1) e += gbl(cptr<T>(F.pw[1]))[K] * xc[r0 + s1];
+  if (j & 1) {
+    e += gbl(cptr<T>(F.pw[2]))[K] * xc[r1 + s0];
+    if (i & 1) e += gbl(cptr<T>(F.pw[3]))[K] * xc[r1 + s1];
+  }
   const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x));
   x[K] = ZERO ? e : x[K] + e;
 }
@@ -1158,7 +1166,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   if (I.sc->status != ST_ACTIVE) return;
   const int lf = fm.lev, nl = I.nlev;
   const bool post_x = smoother & 4, post_y = smoother & 8;
-  const int rows = post_x ? 1 : post_y ? 2 : 0;
+  const int rows = prolong_rows(smoother);
   for (int m = lf; m + 1 < nl; ++m) {   // b_{m+1} = R b_m (x_m = 0)
     const int nc = I.lev[m + 1].nx * I.lev[m + 1].ny;
     for (int KC = threadIdx.x; KC < nc; KC += blockDim.x) restrict_point<T, true>(I, m, xid, bid, KC);
@@ -1464,20 +1472,37 @@ enum Finish { F_INIT64 = 0, F_V, F_S, F_T, F_X, F_IR_RES, F_INNER_INIT, F_FINALI
 
 template <typename A, int NV>
 __device__ __forceinline__ void sum_partials(const InstDev &I, A (&tot)[NV]) {
+  // Thread t adds partials t, t + kFinThreads, .. in that order; the loads of
+  // 8 of them are issued before they are added, so a large instance (65536
+  // partials at 4096^2) costs a few memory latencies, not one per partial.
+  constexpr int U = 8;
   A acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = A(0);
-  for (int b = threadIdx.x; b < I.nblk0; b += kBlock) {
+  const int nb = I.nblk0;
+  int b = threadIdx.x;
+  for (; b + (U - 1) * kFinThreads < nb; b += U * kFinThreads) {
+    A v[NV][U];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] += (A)I.partial[(size_t)k * I.nblk0 + b];
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[k][u] = (A)I.partial[(size_t)k * nb + b + u * kFinThreads];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) acc[k] += v[k][u];
   }
-  block_sum<A, NV>(acc);
+  for (; b < nb; b += kFinThreads) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += (A)I.partial[(size_t)k * nb + b];
+  }
+  block_sum<A, NV, kFinThreads>(acc);
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = acc[k];   // valid in thread 0
 }
 
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ insts, int n,
+__global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restrict__ insts, int n,
                                                   int *__restrict__ cnt) {
   const int k = blockIdx.x;
   if (k >= n) return;
@@ -1589,13 +1614,21 @@ __global__ void __launch_bounds__(kBlock) k_finish(const InstDev *__restrict__ i
   } else {   // F_FINALIZE
     double t[3];
     sum_partials<double, 3>(I, t);
-    __shared__ double sm[kBlock];
+    __shared__ double sm[kFinThreads];
     double mm = DBL_MAX;
-    for (int b = threadIdx.x; b < I.nblk0; b += kBlock)
-      mm = fmin(mm, I.partial[(size_t)3 * I.nblk0 + b]);
+    const double *pmin = I.partial + (size_t)3 * I.nblk0;
+    int b = threadIdx.x;
+    for (; b + 7 * kFinThreads < I.nblk0; b += 8 * kFinThreads) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = pmin[b + u * kFinThreads];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mm = fmin(mm, v[u]);
+    }
+    for (; b < I.nblk0; b += kFinThreads) mm = fmin(mm, pmin[b]);
     sm[threadIdx.x] = mm;
     __syncthreads();
-    for (int o = kBlock / 2; o > 0; o >>= 1) {
+    for (int o = kFinThreads / 2; o > 0; o >>= 1) {
       if (threadIdx.x < o) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + o]);
       __syncthreads();
     }
@@ -1781,14 +1814,14 @@ template <typename T>
 void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt) {
   if (!n) return;
   switch (mode) {
-    case F_INIT64: k_finish<T, F_INIT64><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_V: k_finish<T, F_V><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_S: k_finish<T, F_S><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_T: k_finish<T, F_T><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_X: k_finish<T, F_X><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_IR_RES: k_finish<T, F_IR_RES><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    case F_INNER_INIT: k_finish<T, F_INNER_INIT><<<n, kBlock, 0, s>>>(d, n, cnt); break;
-    default: k_finish<T, F_FINALIZE><<<n, kBlock, 0, s>>>(d, n, cnt); break;
+    case F_INIT64: k_finish<T, F_INIT64><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_V: k_finish<T, F_V><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_S: k_finish<T, F_S><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_T: k_finish<T, F_T><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_X: k_finish<T, F_X><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_IR_RES: k_finish<T, F_IR_RES><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_INNER_INIT: k_finish<T, F_INNER_INIT><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    default: k_finish<T, F_FINALIZE><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
   }
 }
 template void launch_finish<double>(hipStream_t, int, const InstDev *, int, int *);

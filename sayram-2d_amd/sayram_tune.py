@@ -11,11 +11,14 @@ that select how the step is launched, and returns the fastest set:
   eagerly (-1).
 * ``line_tiles``: the line lengths at which the line solves switch tile shape
   (rows: short / medium limits; columns: short / medium limits).
+* ``streams``: the tile shapes of a sweep on two HIP streams (2) or one (1).
+* ``fuse_cells``: the size below which an instance's coarse levels run in the
+  fused one-workgroup cycle (-1: never).
 
-``check_every`` and ``use_graph`` never change results: every instance's
-result is bit-identical for all their values. ``line_tiles`` changes the
-rounding of the line solves (not their accuracy), so the tuned values are part
-of the options a run is reproduced with.
+``check_every``, ``use_graph`` and ``streams`` never change results: every
+instance's result is bit-identical for all their values. ``line_tiles`` and
+``fuse_cells`` change the rounding of the line solves (not their accuracy), so
+the tuned values are part of the options a run is reproduced with.
 
 Each trial builds a fresh batch, so no state is carried between trials. It runs
 ``warmup`` steps, then times ``steps`` steps. The search is coordinate descent
@@ -28,7 +31,7 @@ import time
 import sayram_gpu as sg
 
 DEFAULTS = {"check_every": 1, "use_graph": 1, "row_short": 24, "row_mid": 160,
-            "col_short": 24, "col_mid": 384}
+            "col_short": 24, "col_mid": 384, "streams": 2, "fuse_cells": 1024}
 
 CANDIDATES = {
     "check_every": [1, 2, 3, 4],
@@ -37,12 +40,15 @@ CANDIDATES = {
     "col_mid": [256, 384, 768],
     "row_short": [16, 24, 32],
     "col_short": [16, 24, 32],
+    "streams": [1, 2],
+    "fuse_cells": [-1, 256, 1024],
 }
 
 
 def opts_for(precision: int, p: dict):
     return sg.make_opts(precision=precision, check_every=p["check_every"], use_graph=p["use_graph"],
-                        line_tiles=(p["row_short"], p["row_mid"], p["col_short"], p["col_mid"]))
+                        line_tiles=(p["row_short"], p["row_mid"], p["col_short"], p["col_mid"]),
+                        streams=p.get("streams", 2), fuse_cells=p.get("fuse_cells", 1024))
 
 
 def measure(problems, precision: int, p: dict, steps: int = 2, warmup: int = 1) -> float:
@@ -61,7 +67,8 @@ def measure(problems, precision: int, p: dict, steps: int = 2, warmup: int = 1) 
 
 
 def tune(problems, precision: int = sg.SG_PC32, steps: int = 2, warmup: int = 1,
-         order=("check_every", "use_graph", "col_mid", "row_mid", "col_short", "row_short"),
+         order=("check_every", "use_graph", "streams", "fuse_cells", "col_mid", "row_mid",
+                "col_short", "row_short"),
          log=None):
     """Coordinate descent from DEFAULTS. Returns (best parameters, best value,
     list of (parameters, cell-steps/s) for every measured point)."""

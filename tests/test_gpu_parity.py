@@ -140,3 +140,22 @@ def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
             assert rel_l2(out[fc][k], out[-1][k]) <= 1e-11
         for fc in out:
             assert rel_l2(out[fc][k], ref) <= TOL
+
+
+@pytest.mark.parametrize("graph", [1, -1], ids=["graph", "eager"])
+def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph):
+    """The tile shapes of a sweep (and of the per-step factorisations) run on
+    two streams by default; they solve disjoint lines, so every result equals
+    the one-stream schedule bit for bit. Line lengths 20, 100, 300 and 500 give
+    sweeps with three tile shapes in both directions."""
+    probs = [sc.seeded_case(nx, ny, seed=120 + k) for k, (nx, ny) in
+             enumerate([(20, 300), (100, 20), (300, 100), (500, 40), (60, 500)])]
+    out = {}
+    for ns in (1, 2):
+        b = gpu_lib.Batch(gpu_lib.make_opts(precision=gpu_lib.SG_PC32, streams=ns, use_graph=graph))
+        slots = [b.add(p) for p in probs]
+        b.step(3)
+        out[ns] = [b.f(s) for s in slots]
+        b.close()
+    for k in range(len(probs)):
+        assert np.array_equal(out[1][k], out[2][k])
