@@ -40,6 +40,13 @@ __host__ __device__ constexpr int tile_lines(int sh) {
   return sh == 0 ? 64 : sh == 1 ? 16 : sh == 2 ? 4 : 16;
 }
 __host__ __device__ constexpr int tile_threads(int sh) { return sh == 3 ? 1024 : 256; }
+// Tiles of the eliminated right-hand side d that a line sweep keeps in LDS
+// (the last ones of each line; earlier tiles are parked in x and read back by
+// the backward pass), for values of tbytes bytes. Sized so that the LDS of a
+// workgroup does not lower the occupancy its registers allow.
+__host__ __device__ constexpr int ring_tiles(int sh, int tbytes) {
+  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 ? 6 : 4) : (sh == 3 ? 1 : 2);
+}
 
 __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
 

@@ -433,11 +433,15 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
   const int cx = Cl.nx, cy = Cl.ny, fx = F.nx, fy = F.ny;
-  const int KC = m.k0 + threadIdx.x;
-  if (KC >= cx * cy) return;
+  const int KN = m.k0 + threadIdx.x;
+  if (KN >= cx * cy) return;
   const int hxf = half_x(fx);
-  int Ic, J;
-  cell_ij(KC, cx, half_x(cx), Ic, J);
+  // Work-items take the coarse points in natural order (Ic fastest), so that
+  // the fine points 2 Ic + fa of neighbouring work-items are neighbouring
+  // slots of one half-row of the split layout (coalesced); the output goes to
+  // the point's slot.
+  const int J = KN / cx, Ic = KN - J * cx;
+  const size_t KC = (size_t)J * cx + xslot(Ic, half_x(cx));
   const T *__restrict__ pw[4] = {cptr<T>(F.pw[0]), cptr<T>(F.pw[1]), cptr<T>(F.pw[2]),
                                  cptr<T>(F.pw[3])};
   constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
@@ -644,10 +648,12 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
 // (d_t = a_t d_{t-1} + b_t forward, x_t = d_t - u_t x_{t+1} backward), so
 // every thread composes the maps of its 4 consecutive steps, a segmented wave
 // scan combines them across the S/4 threads of a line, and the carry from the
-// previous tile enters through the composed map. The forward pass parks the
-// eliminated right-hand side in x (no other line of this colour reads it);
-// the backward pass turns it into the solution. ZERO: the other lines are
-// zero (first sweep of a cycle).
+// previous tile enters through the composed map. The eliminated right-hand
+// side d of the last ring_tiles tiles of each line stays in LDS, in a ring of
+// tile buffers (tile t in buffer t mod KT); when a buffer is reused, the d it
+// holds is parked in x (no other line of this colour reads it). The backward
+// pass turns d into the solution, reading back only the parked tiles. ZERO:
+// the other lines are zero (first sweep of a cycle).
 template <typename T, int NB, int DIR, bool ZERO, int SH>
 __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__restrict__ insts,
                                                            const LineGrp *__restrict__ map, int l,
@@ -655,10 +661,12 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   constexpr int S = tile_steps(SH), L = tile_lines(SH), NT = tile_threads(SH);
   constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
   constexpr int Q = S * L / NT;      // elements per thread in the loads
+  constexpr int KT = ring_tiles(SH, sizeof(T));   // tiles of d kept in LDS
+  constexpr int TB = L * (S + 4);                  // one tile buffer
   static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
-  __shared__ T s_r[L * (S + 4)];
-  __shared__ T s_a[L * (S + 4)];
-  __shared__ T s_i[L * (S + 4)];
+  __shared__ T s_rb[KT * TB];   // r, then d, then x of a tile, in a ring of KT
+  __shared__ T s_a[TB];
+  __shared__ T s_i[TB];
   __shared__ T s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
@@ -736,8 +744,19 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   };
   if (tid < L) s_c[tid] = T(0);
   load_fwd(0);
-  for (int t0 = 0; t0 < n; t0 += S) {
+  for (int t0 = 0, tt = 0; t0 < n; t0 += S, ++tt) {
     const int tw = min(S, n - t0);
+    T *const s_r = s_rb + (tt % KT) * TB;
+    if (tt >= KT) {
+      // this buffer holds d of tile tt - KT (a full tile): park it in x. Each
+      // element is read here, and rewritten below, by the thread that loads
+      // it back in the backward pass.
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = ek(q), u = eu(q);
+        if (k < cnt) st32(x, at(gr.first + k * gr.step, t0 - KT * S + u), s_r[tslot<S>(k, u)]);
+      }
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       T r = vb[q] - vom[q] * xm[q] - vop[q] * xp[q];
@@ -783,16 +802,6 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
       if (sk_ok && sc == W - 1) s_c[sk] = d;
     }
     lds_barrier();
-    // The last tile's d stays in LDS: the backward pass starts there.
-    if (t0 + S < n) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int k = ek(q), u = eu(q);
-        if (k < cnt && u < tw)
-          st32(x, at(gr.first + k * gr.step, t0 + u), s_r[tslot<S>(k, u)]);
-      }
-    }
-    lds_barrier();
   }
   T vd[Q], vc[Q];
   auto load_bwd = [&](int t0, bool with_d) {
@@ -810,18 +819,21 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   };
   if (tid < L) s_c[tid] = T(0);
   const int ntiles = (n + S - 1) / S;
-  // d of every earlier tile was stored above by the thread that loads it here
+  // tiles from `resident` on are still in their buffers; the d of every
+  // earlier tile was parked above by the thread that loads it here
+  const int resident = ntiles > KT ? ntiles - KT : 0;
   load_bwd((ntiles - 1) * S, false);
   for (int tt = ntiles - 1; tt >= 0; --tt) {
     const int t0 = tt * S;
     const int tw = min(S, n - t0);
+    T *const s_r = s_rb + (tt % KT) * TB;   // free once its resident tile was done
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      if (tt < ntiles - 1) s_r[tslot<S>(ek(q), eu(q))] = vd[q];
+      if (tt < resident) s_r[tslot<S>(ek(q), eu(q))] = vd[q];
       s_a[tslot<S>(ek(q), eu(q))] = vc[q];
     }
     // prefetch the tile below (this iteration stores tile tt only)
-    if (tt > 0) load_bwd(t0 - S, true);
+    if (tt > 0) load_bwd(t0 - S, tt - 1 < resident);
     lds_barrier();
     {   // backward: x_t = -u_t x_{t+1} + d_t, composed from the line's end
       T al[4], be[4];
