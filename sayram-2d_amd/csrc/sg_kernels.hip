@@ -877,43 +877,63 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   }
 }
 
-// (A x)(i, j) at slot K; kw, ke are the slots of (i-1, j) and (i+1, j)
+// (A x)(i, j) at slot K. Every operand is loaded before the sum, so that a
+// work-item waits for memory once (with the loads behind the boundary tests
+// the compiler issues one round trip per neighbour): neighbour slots are
+// clamped into the grid and the coupling of a neighbour outside it is taken
+// as zero (fma(0, x, s) == s). The terms are added in the stencil's order:
+// centre, west, east, south (south-west, south-east), north (north-west,
+// north-east).
 template <typename T, int NB>
 __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j, const T *xp) {
   const int nx = L.nx, ny = L.ny, hx = half_x(nx);
-  const size_t kw = (size_t)j * nx + xslot(i - 1, hx), ke = (size_t)j * nx + xslot(i + 1, hx);
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+  const size_t row = (size_t)j * nx;
+  const size_t kw = row + xslot(hw ? i - 1 : i, hx), ke = row + xslot(he ? i + 1 : i, hx);
+  const size_t ds = hs ? nx : 0, dn = hn ? nx : 0;
   const auto x = gbl(xp);
-  T s = gbl(cptr<T>(L.band[BC]))[K] * x[K];
-  if (i > 0) s += gbl(cptr<T>(L.band[BW]))[K] * x[kw];
-  if (i < nx - 1) s += gbl(cptr<T>(L.band[BE]))[K] * x[ke];
-  if (j > 0) {
-    s += gbl(cptr<T>(L.band[BS]))[K] * x[K - nx];
-    if (NB == 9) {
-      if (i > 0) s += gbl(cptr<T>(L.band[BSW]))[K] * x[kw - nx];
-      if (i < nx - 1) s += gbl(cptr<T>(L.band[BSE]))[K] * x[ke - nx];
-    }
+  auto band = [&](int b) { return gbl(cptr<T>(L.band[b]))[K]; };
+  const T ac = band(BC), aw = band(BW), ae = band(BE), as = band(BS), an = band(BN);
+  const T x0 = x[K], xw = x[kw], xe = x[ke], xs = x[K - ds], xn = x[K + dn];
+  T asw = T(0), ase = T(0), anw = T(0), ane = T(0), xsw = T(0), xse = T(0), xnw = T(0), xne = T(0);
+  if (NB == 9) {
+    asw = band(BSW); ase = band(BSE); anw = band(BNW); ane = band(BNE);
+    xsw = x[kw - ds]; xse = x[ke - ds]; xnw = x[kw + dn]; xne = x[ke + dn];
   }
-  if (j < ny - 1) {
-    s += gbl(cptr<T>(L.band[BN]))[K] * x[K + nx];
-    if (NB == 9) {
-      if (i > 0) s += gbl(cptr<T>(L.band[BNW]))[K] * x[kw + nx];
-      if (i < nx - 1) s += gbl(cptr<T>(L.band[BNE]))[K] * x[ke + nx];
-    }
+  T s = ac * x0;
+  s += (hw ? aw : T(0)) * xw;
+  s += (he ? ae : T(0)) * xe;
+  s += (hs ? as : T(0)) * xs;
+  if (NB == 9) {
+    s += (hs && hw ? asw : T(0)) * xsw;
+    s += (hs && he ? ase : T(0)) * xse;
+  }
+  s += (hn ? an : T(0)) * xn;
+  if (NB == 9) {
+    s += (hn && hw ? anw : T(0)) * xnw;
+    s += (hn && he ? ane : T(0)) * xne;
   }
   return s;
 }
 
 // M x at slot K with the FP64 level-0 bands (x in FP64, or an FP32 vector
-// promoted on load)
+// promoted on load); loads first, as in stencil_apply
 template <typename X>
 __device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int j, const X *xp) {
   const int nx = I.nx, hx = half_x(nx);
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < I.ny - 1;
+  const size_t row = (size_t)j * nx;
+  const size_t kw = row + xslot(hw ? i - 1 : i, hx), ke = row + xslot(he ? i + 1 : i, hx);
   const auto x = gbl(xp);
-  double s = gbl(I.band64[BC])[K] * (double)x[K];
-  if (i > 0) s += gbl(I.band64[BW])[K] * (double)x[(size_t)j * nx + xslot(i - 1, hx)];
-  if (i < nx - 1) s += gbl(I.band64[BE])[K] * (double)x[(size_t)j * nx + xslot(i + 1, hx)];
-  if (j > 0) s += gbl(I.band64[BS])[K] * (double)x[K - nx];
-  if (j < I.ny - 1) s += gbl(I.band64[BN])[K] * (double)x[K + nx];
+  const double ac = gbl(I.band64[BC])[K], aw = gbl(I.band64[BW])[K], ae = gbl(I.band64[BE])[K];
+  const double as = gbl(I.band64[BS])[K], an = gbl(I.band64[BN])[K];
+  const double x0 = (double)x[K], xw = (double)x[kw], xe = (double)x[ke];
+  const double xs = (double)x[hs ? K - nx : K], xn = (double)x[hn ? K + nx : K];
+  double s = ac * x0;
+  s += (hw ? aw : 0.0) * xw;
+  s += (he ? ae : 0.0) * xe;
+  s += (hs ? as : 0.0) * xs;
+  s += (hn ? an : 0.0) * xn;
   return s;
 }
 
@@ -955,22 +975,33 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
   const auto r = gbl(FROM_B ? b : cptr<T>(F.cp));
   const auto pw0 = gbl(cptr<T>(F.pw[0])), pw1 = gbl(cptr<T>(F.pw[1]));
   const auto pw2 = gbl(cptr<T>(F.pw[2])), pw3 = gbl(cptr<T>(F.pw[3]));
-  T s = T(0);
+  // All 18 loads are issued before the sum (fine points outside the grid are
+  // clamped in and weighted by zero), then added in row-major order.
+  T wv[3][3], rv[3][3];
+  bool ok[3][3];
 #pragma unroll
   for (int db = -1; db <= 1; ++db) {
     const int fj = 2 * J + db;
-    if (fj < 0 || fj >= F.ny) continue;
+    const bool okj = fj >= 0 && fj < F.ny;
+    const size_t rowf = (size_t)(okj ? fj : 2 * J) * F.nx;
 #pragma unroll
     for (int da = -1; da <= 1; ++da) {
       const int fi = 2 * Ic + da;
-      if (fi < 0 || fi >= F.nx) continue;
+      const bool oki = fi >= 0 && fi < F.nx;
+      const size_t kf = rowf + xslot(oki ? fi : 2 * Ic, hxf);
       // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
       const int slot = (da < 0 ? 1 : 0) + 2 * (db < 0 ? 1 : 0);
-      const size_t kf = (size_t)fj * F.nx + xslot(fi, hxf);
       const auto w = slot == 0 ? pw0 : slot == 1 ? pw1 : slot == 2 ? pw2 : pw3;
-      s += w[kf] * r[kf];
+      wv[db + 1][da + 1] = w[kf];
+      rv[db + 1][da + 1] = r[kf];
+      ok[db + 1][da + 1] = okj && oki;
     }
   }
+  T s = T(0);
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s += (ok[b][a] ? wv[b][a] : T(0)) * rv[b][a];
   gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
 }
 
@@ -1027,12 +1058,28 @@ __device__ __forceinline__ void prolong_point(const InstDev &I, int l, int xid, 
   // Weights of parents outside the coarse grid are zero, and so are the
   // weights to the parents at I0 + 1 of an even i and at J0 + 1 of an even j
   // (k_pweights writes exact zeros there): those terms are skipped, so half
-  // of each row's weight runs is not read.
-  T e = gbl(cptr<T>(F.pw[0]))[K] * xc[r0 + s0];
-  if (i & 1) e += gbl(cptr<T>(F.pw[1]))[K] * xc[r0 + s1];
-  if (j & 1) {
-    e += gbl(cptr<T>(F.pw[2]))[K] * xc[r1 + s0];
-    if (i & 1) e += gbl(cptr<T>(F.pw[3]))[K] * xc[r1 + s1];
+  // of each row's weight runs is not read. Each parity case issues all its
+  // loads before it adds (i's parity is uniform over each half-row).
+  const auto w0 = gbl(cptr<T>(F.pw[0])), w1 = gbl(cptr<T>(F.pw[1]));
+  const auto w2 = gbl(cptr<T>(F.pw[2])), w3 = gbl(cptr<T>(F.pw[3]));
+  T e;
+  if ((i & 1) && (j & 1)) {
+    const T a0 = w0[K], a1 = w1[K], a2 = w2[K], a3 = w3[K];
+    const T c0 = xc[r0 + s0], c1 = xc[r0 + s1], c2 = xc[r1 + s0], c3 = xc[r1 + s1];
+    e = a0 * c0;
+    e += a1 * c1;
+    e += a2 * c2;
+    e += a3 * c3;
+  } else if (i & 1) {
+    const T a0 = w0[K], a1 = w1[K], c0 = xc[r0 + s0], c1 = xc[r0 + s1];
+    e = a0 * c0;
+    e += a1 * c1;
+  } else if (j & 1) {
+    const T a0 = w0[K], a2 = w2[K], c0 = xc[r0 + s0], c2 = xc[r1 + s0];
+    e = a0 * c0;
+    e += a2 * c2;
+  } else {
+    e = w0[K] * xc[r0 + s0];
   }
   const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x));
   x[K] = ZERO ? e : x[K] + e;
