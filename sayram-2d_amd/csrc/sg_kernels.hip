@@ -74,6 +74,13 @@ template <typename T> __device__ __forceinline__ const T *cptr(const void *p) {
   return static_cast<const T *>(p);
 }
 
+// Read-only view of an instance's scalars, for the kernels that do not write
+// them (only the begin and finish kernels do). Through the constant address
+// space the compiler reads these workgroup-uniform values with scalar loads
+// instead of per-lane loads that every later load waits behind.
+typedef const __attribute__((address_space(4))) KScal ConstKScal;
+__device__ __forceinline__ ConstKScal *rsc(const InstDev &I) { return (ConstKScal *)I.sc; }
+
 // ---------------------------------------------------------------------------
 // workgroup reduction of NT threads: wave64 shuffles, then LDS across the
 // NT / 64 waves
@@ -250,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
 #pragma clang fp contract(off)
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const int nx = I.nx, ny = I.ny;
   const int K = m.k0 + threadIdx.x;
   if (K >= nx * ny) return;
@@ -325,7 +332,7 @@ __global__ void __launch_bounds__(kBlock) k_bands_to_f32(const InstDev *__restri
                                                         const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const int K = m.k0 + threadIdx.x;
   if (K >= I.nx * I.ny) return;
 #pragma unroll
@@ -385,7 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
                                                     const BlkMap *__restrict__ map, int l) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const Lev &L = I.lev[l];
   const int nx = L.nx, ny = L.ny;
   const int K = m.k0 + threadIdx.x;
@@ -429,7 +436,7 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
                                                const BlkMap *__restrict__ map, int l) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
   const int cx = Cl.nx, cy = Cl.ny, fx = F.nx, fy = F.ny;
@@ -525,7 +532,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
   __shared__ double s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const Lev &Lv = I.lev[l];
   const int nx = Lv.nx, ny = Lv.ny;
   const double *__restrict__ bc = Lv.band64[BC];
@@ -670,7 +677,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   __shared__ T s_c[L];
   const LineGrp gr = map[blockIdx.x];
   const InstDev &I = insts[gr.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const Lev &Lv = I.lev[l];
   const int nx = Lv.nx, ny = Lv.ny;
   T *x; const T *b;
@@ -946,7 +953,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
                                                     int xid, int bid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const Lev &F = I.lev[l];
   const int K = m.k0 + threadIdx.x;
   if (K >= F.nx * F.ny) return;
@@ -1011,7 +1018,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
                                                     int xid, int bid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int KC = m.k0 + threadIdx.x;
   if (KC >= I.lev[l + 1].nx * I.lev[l + 1].ny) return;
   restrict_point<T, FROM_B>(I, l, xid, bid, KC);
@@ -1091,7 +1098,7 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
                                                    int xid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int idx = m.k0 + threadIdx.x;
   if (idx >= prolong_count<ROWS>(I.lev[l])) return;
   prolong_point<T, ZERO, ROWS>(I, l, xid, idx);
@@ -1222,7 +1229,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   __shared__ FuseLds<T> S;
   const FuseMap fm = map[blockIdx.x];
   const InstDev &I = insts[fm.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int lf = fm.lev, nl = I.nlev;
   const bool post_x = smoother & 4, post_y = smoother & 8;
   const int rows = prolong_rows(smoother);
@@ -1300,7 +1307,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
                                                        const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   double v[2] = {0.0, 0.0};
@@ -1324,7 +1331,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restric
 // in FP32 range however far the residual has fallen).
 template <typename T, bool PC>
 __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, size_t K) {
-  if (PC) return (T)(I.sc->pcs * (double)gbl(I.pv[pcid - kNumVec])[K]);
+  if (PC) return (T)(rsc(I)->pcs * (double)gbl(I.pv[pcid - kNumVec])[K]);
   return gbl(cptr<T>(I.kv[id]))[K];
 }
 
@@ -1334,14 +1341,15 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   if (K >= I.nx * I.ny) return;
-  const T beta = (T)I.sc->beta, omega = (T)I.sc->omega;
+  const T beta = (T)rsc(I)->beta, omega = (T)rsc(I)->omega;
+  const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
   T *p = ptr<T>(I.kv[VP]);
   const T pv = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
   gbl(p)[K] = pv;
-  if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(pv / I.sc->pcs);
+  if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(pv / pcs);
 }
 
 // v = M ph; partial (rh, v)
@@ -1350,17 +1358,18 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   T acc[1] = {T(0)};
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const T v = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_PH - kNumVec])))
+    const T rh = gbl(cptr<T>(I.kv[VRH]))[K];   // loaded before the store below
+    const T v = PC ? (T)(rsc(I)->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_PH - kNumVec])))
                    : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
     gbl(ptr<T>(I.kv[VV]))[K] = v;
-    acc[0] = gbl(cptr<T>(I.kv[VRH]))[K] * v;
+    acc[0] = rh * v;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
@@ -1371,14 +1380,15 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ i
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   T acc[1] = {T(0)};
   if (K < N) {
-    const T s = gbl(cptr<T>(I.kv[VR]))[K] - (T)I.sc->alpha * gbl(cptr<T>(I.kv[VV]))[K];
+    const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
+    const T s = gbl(cptr<T>(I.kv[VR]))[K] - (T)rsc(I)->alpha * gbl(cptr<T>(I.kv[VV]))[K];
     gbl(ptr<T>(I.kv[VS]))[K] = s;
-    if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(s / I.sc->pcs);
+    if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(s / pcs);
     acc[0] = s * s;
   }
   store_partials<T, 1>(I, m.k0 / kBlock, acc);
@@ -1390,17 +1400,18 @@ __global__ void __launch_bounds__(kBlock) k_bicg_t(const InstDev *__restrict__ i
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->status != ST_ACTIVE) return;
+  if (rsc(I)->status != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   T acc[2] = {T(0), T(0)};
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const T t = PC ? (T)(I.sc->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_SH - kNumVec])))
+    const T sv = gbl(cptr<T>(I.kv[VS]))[K];   // loaded before the store below
+    const T t = PC ? (T)(rsc(I)->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_SH - kNumVec])))
                    : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VSH]));
     gbl(ptr<T>(I.kv[VT]))[K] = t;
-    acc[0] = t * gbl(cptr<T>(I.kv[VS]))[K];
+    acc[0] = t * sv;
     acc[1] = t * t;
   }
   store_partials<T, 2>(I, m.k0 / kBlock, acc);
@@ -1414,24 +1425,31 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ i
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  const int st = I.sc->status;
+  const int st = rsc(I)->status;
   if (st != ST_ACTIVE && st != ST_HALF) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
-  const T alpha = (T)I.sc->alpha, omega = (T)I.sc->omega;
+  const T alpha = (T)rsc(I)->alpha, omega = (T)rsc(I)->omega;
   T acc[2] = {T(0), T(0)};
   if (K < N) {
+    // every load before the first store (the stores could alias the later
+    // loads for the compiler, which would then wait for each in turn)
+    // (an instance that converged at s reads sh, s, t and rh needlessly, once;
+    // they enter both cases through selects so that their loads are issued
+    // with the others)
     T *x = XSTATE ? (T *)I.x : ptr<T>(I.kv[VXK]);
+    const T xv = gbl(x)[K];
     const T ph = precond_vec<T, PC>(I, VPH, PV_PH, K);
-    if (st == ST_HALF) {
-      gbl(x)[K] = gbl(x)[K] + alpha * ph;
-    } else {
-      gbl(x)[K] = gbl(x)[K] + alpha * ph + omega * precond_vec<T, PC>(I, VSH, PV_SH, K);
-      const T r = gbl(cptr<T>(I.kv[VS]))[K] - omega * gbl(cptr<T>(I.kv[VT]))[K];
-      gbl(ptr<T>(I.kv[VR]))[K] = r;
-      acc[0] = gbl(cptr<T>(I.kv[VRH]))[K] * r;
-      acc[1] = r * r;
-    }
+    const T sh = precond_vec<T, PC>(I, VSH, PV_SH, K);
+    const T sv = gbl(cptr<T>(I.kv[VS]))[K], tv = gbl(cptr<T>(I.kv[VT]))[K];
+    const T rh = gbl(cptr<T>(I.kv[VRH]))[K];
+    const bool act = st != ST_HALF;
+    const T xh = xv + alpha * ph;
+    const T r = sv - omega * tv;
+    gbl(x)[K] = act ? xh + omega * sh : xh;
+    acc[0] = act ? rh * r : T(0);
+    acc[1] = act ? r * r : T(0);
+    if (act) gbl(ptr<T>(I.kv[VR]))[K] = r;
   }
   if (st == ST_ACTIVE) store_partials<T, 2>(I, m.k0 / kBlock, acc);
 }
@@ -1441,7 +1459,7 @@ __global__ void __launch_bounds__(kBlock) k_ir_residual(const InstDev *__restric
                                                        const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus != ST_ACTIVE) return;
+  if (rsc(I)->ostatus != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   double v[2] = {0.0, 0.0};
@@ -1461,12 +1479,12 @@ __global__ void __launch_bounds__(kBlock) k_ir_inner_init(const InstDev *__restr
                                                          const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus != ST_ACTIVE) return;
+  if (rsc(I)->ostatus != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   float v[1] = {0.0f};
   if (K < N) {
-    const float r = (float)(gbl(I.r64)[K] / I.sc->scale);
+    const float r = (float)(gbl(I.r64)[K] / rsc(I)->scale);
     gbl(ptr<float>(I.kv[VR]))[K] = r;
     gbl(ptr<float>(I.kv[VRH]))[K] = r;
     gbl(ptr<float>(I.kv[VP]))[K] = 0.0f;
@@ -1482,10 +1500,10 @@ __global__ void __launch_bounds__(kBlock) k_ir_update(const InstDev *__restrict_
                                                      const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus != ST_ACTIVE) return;
+  if (rsc(I)->ostatus != ST_ACTIVE) return;
   const int K = m.k0 + threadIdx.x;
   if (K >= I.nx * I.ny) return;
-  gbl(I.x)[K] += I.sc->scale * (double)gbl(ptr<float>(I.kv[VXK]))[K];
+  gbl(I.x)[K] += rsc(I)->scale * (double)gbl(ptr<float>(I.kv[VXK]))[K];
 }
 
 // f = max(x, 0); partials of the true residual of the unclamped x, ||R||, the
@@ -1494,7 +1512,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
                                                     const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (I.sc->ostatus == ST_IDLE) return;
+  if (rsc(I)->ostatus == ST_IDLE) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
   double v[3] = {0.0, 0.0, 0.0};
