@@ -147,54 +147,60 @@ __global__ void k_copy_counter(const int *__restrict__ src, int *__restrict__ ds
 // ---------------------------------------------------------------------------
 
 // Vertex value (fill_vertex_from_cells + fill_vertex_from_bcs,
-// source/Solver.cc:299-422). Dirichlet sides were written in the order XMIN,
-// XMAX, YMIN, YMAX there, so the precedence here is the reverse.
-__device__ double vertex_value(const InstDev &I, const double *__restrict__ f, int vi, int vj) {
+// source/Solver.cc:299-422) of vertex (vi, vj) from the f of the four cells
+// around it, fSW = (vi-1, vj-1), fSE = (vi, vj-1), fNW = (vi-1, vj),
+// fNE = (vi, vj) (loaded up front; only cells inside the grid are used here),
+// and wxL/wxR = wx0/wx1[vi], wyB/wyT = wy0/wy1[vj]. Dirichlet sides were
+// written in the order XMIN, XMAX, YMIN, YMAX there, so the precedence here
+// is the reverse.
+__device__ __forceinline__ double vertex_value(const InstDev &I, int vi, int vj, double fSW,
+                                               double fSE, double fNW, double fNE, double wxL,
+                                               double wxR, double wyB, double wyT) {
 #pragma clang fp contract(off)
-  const int nx = I.nx, ny = I.ny, hx = half_x(nx);
-  auto F = [&](int ci, int cj) { return f[(size_t)cj * nx + xslot(ci, hx)]; };
+  const int nx = I.nx, ny = I.ny;
   if (vj == ny && (I.bcmask & 8)) return I.bcv[3][vi];
   if (vj == 0 && (I.bcmask & 4)) return I.bcv[2][vi];
   if (vi == nx && (I.bcmask & 2)) return I.bcv[1][vj];
   if (vi == 0 && (I.bcmask & 1)) return I.bcv[0][vj];
   const bool bx = (vi == 0 || vi == nx), by = (vj == 0 || vj == ny);
+  // the cell column inside the grid on a vertical boundary (x = 0: the
+  // vertex's east cells), the row inside on a horizontal one
+  const bool east = vi == 0, north = vj == 0;
   if (bx && by) {   // corners copy the adjacent cell (:316-319)
-    const int ci = vi == 0 ? 0 : nx - 1, cj = vj == 0 ? 0 : ny - 1;
-    return F(ci, cj);
+    return north ? (east ? fNE : fNW) : (east ? fSE : fSW);
   }
   if (bx) {         // vertical boundary edge: linear in y (:355-366)
-    const int ci = vi == 0 ? 0 : nx - 1;
-    return I.wy0[vj] * F(ci, vj - 1) + I.wy1[vj] * F(ci, vj);
+    return east ? wyB * fSE + wyT * fNE : wyB * fSW + wyT * fNW;
   }
   if (by) {         // horizontal boundary edge: linear in x (:371-381)
-    const int cj = vj == 0 ? 0 : ny - 1;
-    return I.wx0[vi] * F(vi - 1, cj) + I.wx1[vi] * F(vi, cj);
+    return north ? wxL * fNW + wxR * fNE : wxL * fSW + wxR * fSE;
   }
-  const double wxL = I.wx0[vi], wxR = I.wx1[vi], wyB = I.wy0[vj], wyT = I.wy1[vj];
-  return wxL * wyB * F(vi - 1, vj - 1) + wxR * wyB * F(vi, vj - 1) + wxL * wyT * F(vi - 1, vj) +
-         wxR * wyT * F(vi, vj);
+  return wxL * wyB * fSW + wxR * wyB * fSE + wxL * wyT * fNW + wxR * wyT * fNE;
 }
+
+// Edges of one cell (i, j): xe[i], xe[i+1], ye[j], ye[j+1].
+struct CellEdges { double x0, x1, y0, y1; };
 
 // a_sigma_func (source/Solver.cc:68-97) with the edge table of
 // Mesh::build_connectivity_ (source/Mesh.cc:67-155). The operations are those
 // of the reference in the same order; |B - A| is written as |dx| + |dy|, which
 // equals sqrt(dx^2 + dy^2) exactly because one of the two is zero.
-__device__ __forceinline__ void a_sigma(const double *__restrict__ xe, const double *__restrict__ ye,
-                                        int i, int j, int inbr, double Lxx, double Lxy, double Lyy,
-                                        double fA, double fB, double &aA, double &aB, double &as) {
+__device__ __forceinline__ void a_sigma(const CellEdges &c, int inbr, double Lxx, double Lxy,
+                                        double Lyy, double fA, double fB, double &aA, double &aB,
+                                        double &as) {
 #pragma clang fp contract(off)
   double Ax, Ay, Bx, By, n0, n1;
   if (inbr == 0) {        // im, XNEG
-    Ax = xe[i]; Ay = ye[j + 1]; Bx = xe[i]; By = ye[j]; n0 = -1.0; n1 = 0.0;
+    Ax = c.x0; Ay = c.y1; Bx = c.x0; By = c.y0; n0 = -1.0; n1 = 0.0;
   } else if (inbr == 1) { // jp, YPOS
-    Ax = xe[i + 1]; Ay = ye[j + 1]; Bx = xe[i]; By = ye[j + 1]; n0 = 0.0; n1 = 1.0;
+    Ax = c.x1; Ay = c.y1; Bx = c.x0; By = c.y1; n0 = 0.0; n1 = 1.0;
   } else if (inbr == 2) { // ip, XPOS
-    Ax = xe[i + 1]; Ay = ye[j]; Bx = xe[i + 1]; By = ye[j + 1]; n0 = 1.0; n1 = 0.0;
+    Ax = c.x1; Ay = c.y0; Bx = c.x1; By = c.y1; n0 = 1.0; n1 = 0.0;
   } else {                // jm, YNEG
-    Ax = xe[i]; Ay = ye[j]; Bx = xe[i + 1]; By = ye[j]; n0 = 0.0; n1 = -1.0;
+    Ax = c.x0; Ay = c.y0; Bx = c.x1; By = c.y0; n0 = 0.0; n1 = -1.0;
   }
-  const double Kx = 0.5 * (xe[i] + xe[i + 1]);
-  const double Ky = 0.5 * (ye[j] + ye[j + 1]);
+  const double Kx = 0.5 * (c.x0 + c.x1);
+  const double Ky = 0.5 * (c.y0 + c.y1);
   const double len = fabs(Bx - Ax) + fabs(By - Ay);
   const double vkbx = Bx - Kx, vkby = By - Ky;
   const double vkax = Ax - Kx, vkay = Ay - Ky;
@@ -212,17 +218,16 @@ __device__ __forceinline__ void a_sigma(const double *__restrict__ xe, const dou
 }
 
 // apply_inner_face_pair (source/Solver.cc:99-141) for the face owned by cell
-// (io, jo): its west face (inbr 0) or south face (inbr 3). vA/vB are the
-// owner's edge vertex values; the neighbour sees the same edge reversed.
-__device__ __forceinline__ void face_pair(const double *__restrict__ xe,
-                                          const double *__restrict__ ye, int io, int jo, int inbr,
+// co: its west face (inbr 0) or south face (inbr 3), with cl the neighbour
+// across it. vA/vB are the owner's edge vertex values; the neighbour sees the
+// same edge reversed.
+__device__ __forceinline__ void face_pair(const CellEdges &co, const CellEdges &cl, int inbr,
                                           const double (&LK)[3], const double (&LL)[3], double fK,
                                           double fL, double vA, double vB, double &AK, double &AL) {
 #pragma clang fp contract(off)
   double aKA, aKB, asK, aLA, aLB, asL;
-  a_sigma(xe, ye, io, jo, inbr, LK[0], LK[1], LK[2], vA, vB, aKA, aKB, asK);
-  const int li = inbr == 0 ? io - 1 : io, lj = inbr == 0 ? jo : jo - 1;
-  a_sigma(xe, ye, li, lj, inbr == 0 ? 2 : 1, LL[0], LL[1], LL[2], vB, vA, aLA, aLB, asL);
+  a_sigma(co, inbr, LK[0], LK[1], LK[2], vA, vB, aKA, aKB, asK);
+  a_sigma(cl, inbr == 0 ? 2 : 1, LL[0], LL[1], LL[2], vB, vA, aLA, aLB, asL);
   const double denom = fabs(asK) + fabs(asL) + 2 * kEps;     // calculate_mu, Solver.h:63-67
   const double muK = (fabs(asL) + kEps) / denom;
   const double muL = 1 - muK;
@@ -234,13 +239,12 @@ __device__ __forceinline__ void face_pair(const double *__restrict__ xe,
 }
 
 // apply_dirichlet_face (source/Solver.cc:143-164)
-__device__ __forceinline__ void dirichlet_face(const double *__restrict__ xe,
-                                               const double *__restrict__ ye, int i, int j,
-                                               int inbr, const double (&LK)[3], double fK,
-                                               double vA, double vB, double &AK, double &Bm) {
+__device__ __forceinline__ void dirichlet_face(const CellEdges &c, int inbr, const double (&LK)[3],
+                                               double fK, double vA, double vB, double &AK,
+                                               double &Bm) {
 #pragma clang fp contract(off)
   double aA, aB, as;
-  a_sigma(xe, ye, i, j, inbr, LK[0], LK[1], LK[2], vA, vB, aA, aB, as);
+  a_sigma(c, inbr, LK[0], LK[1], LK[2], vA, vB, aA, aB, as);
   const double Bs = -as;
   const double Ba = fabs(Bs);
   const double Bp = (Ba + Bs) / 2.0;
@@ -252,6 +256,11 @@ __device__ __forceinline__ void dirichlet_face(const double *__restrict__ xe,
 // is summed in the order the reference pushes its triplets (west pairs, south
 // pairs, Dirichlet XMIN/XMAX/YMIN/YMAX, time term), which is the order Eigen's
 // setFromTriplets adds duplicates; R likewise. Also seeds x = f (warm start).
+// Every array value the cell needs is loaded first, with neighbour indices
+// clamped into the grid (values of neighbours outside it are loaded but never
+// used), so that a work-item waits for memory once rather than behind each
+// boundary test; only the Dirichlet vertex arrays, read by boundary cells, are
+// loaded where they are used.
 __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map) {
 #pragma clang fp contract(off)
@@ -264,59 +273,78 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
   const int hx = half_x(nx);
   int i, j;
   cell_ij(K, nx, hx, i, j);
-  const size_t kw = (size_t)j * nx + xslot(i - 1, hx), ke = (size_t)j * nx + xslot(i + 1, hx);
-  const double *__restrict__ f = I.f;
-  const double *xe = I.xe, *ye = I.ye;
-  const double vSW = vertex_value(I, f, i, j), vSE = vertex_value(I, f, i + 1, j);
-  const double vNW = vertex_value(I, f, i, j + 1), vNE = vertex_value(I, f, i + 1, j + 1);
-  const double LK[3] = {I.Lxx[K], I.Lxy[K], I.Lyy[K]};
-  const double fK = f[K];
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+  const size_t r0 = (size_t)j * nx, rm = hs ? r0 - nx : r0, rp = hn ? r0 + nx : r0;
+  const int sm = xslot(hw ? i - 1 : i, hx), s0 = xslot(i, hx), sp = xslot(he ? i + 1 : i, hx);
+  const auto f = gbl(I.f);
+  const double f3[3][3] = {{f[rm + sm], f[rm + s0], f[rm + sp]},
+                           {f[r0 + sm], f[K], f[r0 + sp]},
+                           {f[rp + sm], f[rp + s0], f[rp + sp]}};
+  const auto Lxx = gbl(I.Lxx), Lxy = gbl(I.Lxy), Lyy = gbl(I.Lyy);
+  const size_t kw = r0 + sm, ke = r0 + sp, ks = rm + s0, kn = rp + s0;
+  const double LK[3] = {Lxx[K], Lxy[K], Lyy[K]};
+  const double LW[3] = {Lxx[kw], Lxy[kw], Lyy[kw]}, LE[3] = {Lxx[ke], Lxy[ke], Lyy[ke]};
+  const double LS[3] = {Lxx[ks], Lxy[ks], Lyy[ks]}, LN[3] = {Lxx[kn], Lxy[kn], Lyy[kn]};
+  const auto xe = gbl(I.xe), ye = gbl(I.ye);
+  const double xm = xe[hw ? i - 1 : i], x0 = xe[i], x1 = xe[i + 1], x2 = xe[he ? i + 2 : i + 1];
+  const double ym = ye[hs ? j - 1 : j], y0 = ye[j], y1 = ye[j + 1], y2 = ye[hn ? j + 2 : j + 1];
+  const auto wx0 = gbl(I.wx0), wx1 = gbl(I.wx1), wy0 = gbl(I.wy0), wy1 = gbl(I.wy1);
+  const double wxLi = wx0[i], wxRi = wx1[i], wxLe = wx0[i + 1], wxRe = wx1[i + 1];
+  const double wyBj = wy0[j], wyTj = wy1[j], wyBn = wy0[j + 1], wyTn = wy1[j + 1];
+  const double GK = gbl(I.G)[K], itK = gbl(I.itau)[K];
+  const CellEdges cK{x0, x1, y0, y1}, cW{xm, x0, y0, y1}, cE{x1, x2, y0, y1};
+  const CellEdges cS{x0, x1, ym, y0}, cN{x0, x1, y1, y2};
+  const double vSW = vertex_value(I, i, j, f3[0][0], f3[0][1], f3[1][0], f3[1][1], wxLi, wxRi,
+                                  wyBj, wyTj);
+  const double vSE = vertex_value(I, i + 1, j, f3[0][1], f3[0][2], f3[1][1], f3[1][2], wxLe, wxRe,
+                                  wyBj, wyTj);
+  const double vNW = vertex_value(I, i, j + 1, f3[1][0], f3[1][1], f3[2][0], f3[2][1], wxLi, wxRi,
+                                  wyBn, wyTn);
+  const double vNE = vertex_value(I, i + 1, j + 1, f3[1][1], f3[1][2], f3[2][1], f3[2][2], wxLe,
+                                  wxRe, wyBn, wyTn);
+  const double fK = f3[1][1];
   double d = 0.0, R = 0.0, bw = 0.0, be = 0.0, bs = 0.0, bn = 0.0, AK, AL;
-  if (i > 0) {   // own west face (west-pair loop, :173-175)
-    const double LL[3] = {I.Lxx[kw], I.Lxy[kw], I.Lyy[kw]};
-    face_pair(xe, ye, i, j, 0, LK, LL, fK, f[kw], vNW, vSW, AK, AL);
+  if (hw) {   // own west face (west-pair loop, :173-175)
+    face_pair(cK, cW, 0, LK, LW, fK, f3[1][0], vNW, vSW, AK, AL);
     d += AK;
     bw = 0.0 + -AL;
   }
-  if (i < nx - 1) {   // east face, owned by (i+1, j): this cell is its L
-    const double LO[3] = {I.Lxx[ke], I.Lxy[ke], I.Lyy[ke]};
-    face_pair(xe, ye, i + 1, j, 0, LO, LK, f[ke], fK, vNE, vSE, AK, AL);
+  if (he) {   // east face, owned by (i+1, j): this cell is its L
+    face_pair(cE, cK, 0, LE, LK, f3[1][2], fK, vNE, vSE, AK, AL);
     d += AL;
     be = 0.0 + -AK;
   }
-  if (j > 0) {   // own south face (south-pair loop, :178-182)
-    const double LL[3] = {I.Lxx[K - nx], I.Lxy[K - nx], I.Lyy[K - nx]};
-    face_pair(xe, ye, i, j, 3, LK, LL, fK, f[K - nx], vSW, vSE, AK, AL);
+  if (hs) {   // own south face (south-pair loop, :178-182)
+    face_pair(cK, cS, 3, LK, LS, fK, f3[0][1], vSW, vSE, AK, AL);
     d += AK;
     bs = 0.0 + -AL;
   }
-  if (j < ny - 1) {   // north face, owned by (i, j+1)
-    const double LO[3] = {I.Lxx[K + nx], I.Lxy[K + nx], I.Lyy[K + nx]};
-    face_pair(xe, ye, i, j + 1, 3, LO, LK, f[K + nx], fK, vNW, vNE, AK, AL);
+  if (hn) {   // north face, owned by (i, j+1)
+    face_pair(cN, cK, 3, LN, LK, f3[2][1], fK, vNW, vNE, AK, AL);
     d += AL;
     bn = 0.0 + -AK;
   }
   double Bm;
   if (i == 0 && (I.bcmask & 1)) {        // XMIN: face im
-    dirichlet_face(xe, ye, i, j, 0, LK, fK, vNW, vSW, AK, Bm);
+    dirichlet_face(cK, 0, LK, fK, vNW, vSW, AK, Bm);
     d += AK; R += Bm;
   }
   if (i == nx - 1 && (I.bcmask & 2)) {   // XMAX: face ip
-    dirichlet_face(xe, ye, i, j, 2, LK, fK, vSE, vNE, AK, Bm);
+    dirichlet_face(cK, 2, LK, fK, vSE, vNE, AK, Bm);
     d += AK; R += Bm;
   }
   if (j == 0 && (I.bcmask & 4)) {        // YMIN: face jm
-    dirichlet_face(xe, ye, i, j, 3, LK, fK, vSW, vSE, AK, Bm);
+    dirichlet_face(cK, 3, LK, fK, vSW, vSE, AK, Bm);
     d += AK; R += Bm;
   }
   if (j == ny - 1 && (I.bcmask & 8)) {   // YMAX: face jp
-    dirichlet_face(xe, ye, i, j, 1, LK, fK, vNE, vNW, AK, Bm);
+    dirichlet_face(cK, 1, LK, fK, vNE, vNW, AK, Bm);
     d += AK; R += Bm;
   }
   // time / loss diagonal (:190-199), cell_area_dt (source/Mesh.h:61-63)
-  const double dx = xe[i + 1] - xe[i], dy = ye[j + 1] - ye[j];
-  const double UKK = I.G[K] * (dx * dy / I.dt);
-  d += UKK * (1 + I.dt * I.itau[K]);
+  const double dx = x1 - x0, dy = y1 - y0;
+  const double UKK = GK * (dx * dy / I.dt);
+  d += UKK * (1 + I.dt * itK);
   R += UKK * fK;
   I.band64[BC][K] = d;
   I.band64[BW][K] = bw;
@@ -409,12 +437,18 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
   } else if (!io && jo) {
     w_yedge<NB>(L, K, j, w0, w2);
   } else {
+    // The east and north neighbours' edge weights are computed at clamped
+    // positions and zeroed when those neighbours lie outside the grid, so
+    // that all five stencils are loaded together.
     const St9 a = load_st<NB>(L, K);
-    double sW = 0, nW = 0, sE = 0, nE = 0, wS = 0, eS = 0, wN = 0, eN = 0;
+    const bool he = i + 1 < nx, hn = j + 1 < ny;
+    double sW, nW, sE, nE, wS, eS, wN, eN;
     w_yedge<NB>(L, (size_t)j * nx + xslot(i - 1, hx), j, sW, nW);
-    if (i + 1 < nx) w_yedge<NB>(L, (size_t)j * nx + xslot(i + 1, hx), j, sE, nE);
+    w_yedge<NB>(L, (size_t)j * nx + xslot(he ? i + 1 : i - 1, hx), j, sE, nE);
     w_xedge<NB>(L, K - nx, i, wS, eS);
-    if (j + 1 < ny) w_xedge<NB>(L, K + nx, i, wN, eN);
+    w_xedge<NB>(L, hn ? K + nx : K - nx, i, wN, eN);
+    if (!he) { sE = 0; nE = 0; }
+    if (!hn) { wN = 0; eN = 0; }
     const double c = a.c;
     w0 = -(a.w * sW + a.s * wS + a.sw) / c;
     w1 = -(a.e * sE + a.s * eS + a.se) / c;
