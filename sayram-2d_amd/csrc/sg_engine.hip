@@ -885,7 +885,6 @@ int Batch::step_once() {
                mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
   const int nb0 = dm.ncmap[0];
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
-  if (h32) launch_bands_to_f32(stream, d_insts, dm.cmap[0], nb0);
   for (int l = 0; l < dm.maxlev; ++l) {
     tbegin(1);
     // LU factors of the line matrices: smoothing lines of both colours and the

@@ -11,6 +11,7 @@
 // even and odd columns). A cell kernel's work-item K is a storage slot; it
 // decodes (i, j) with cell_ij and finds its x-neighbours with xslot.
 #include "sg_kernels.hpp"
+#include "sayram_gpu.h"   // sg_precision values of InstDev::prec
 
 #include <cfloat>
 
@@ -351,20 +352,16 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
   I.band64[BE][K] = be;
   I.band64[BS][K] = bs;
   I.band64[BN][K] = bn;
+  if (I.prec != SG_FP64) {   // FP32 copies for the FP32 V-cycle and Krylov forms
+    const Lev &L0 = I.lev[0];
+    ptr<float>(L0.band[BC])[K] = (float)d;
+    ptr<float>(L0.band[BW])[K] = (float)bw;
+    ptr<float>(L0.band[BE])[K] = (float)be;
+    ptr<float>(L0.band[BS])[K] = (float)bs;
+    ptr<float>(L0.band[BN])[K] = (float)bn;
+  }
   I.R[K] = R;
   I.x[K] = fK;
-}
-
-// FP64 level-0 bands -> FP32 copies (mixed precision)
-__global__ void __launch_bounds__(kBlock) k_bands_to_f32(const InstDev *__restrict__ insts,
-                                                        const BlkMap *__restrict__ map) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (rsc(I)->ostatus == ST_IDLE) return;
-  const int K = m.k0 + threadIdx.x;
-  if (K >= I.nx * I.ny) return;
-#pragma unroll
-  for (int b = 0; b < 5; ++b) ptr<float>(I.lev[0].band[b])[K] = (float)I.band64[b][K];
 }
 
 // ---------------------------------------------------------------------------
@@ -1212,13 +1209,17 @@ __device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool z
   __syncthreads();
   for (int q = threadIdx.x; q < nl; q += blockDim.x) {
     const int li = colour + 2 * q;
+    // (unrolled so that the LDS reads of several steps, which do not depend
+    // on the recurrence, are issued together)
     T d = T(0);
+#pragma unroll 8
     for (int t = 0; t < n; ++t) {   // forward: d_t = (r_t - lo_t d_{t-1}) / pivot_t
       const uint32_t K = at(li, t);
       d = (S.r[K] - S.lo[K] * d) * S.inv[K];
       S.r[K] = d;
     }
     T xn = T(0);
+#pragma unroll 8
     for (int t = n - 1; t >= 0; --t) {   // backward: x_t = d_t - u_t x_{t+1}
       const uint32_t K = at(li, t);
       xn = S.r[K] - S.up[K] * xn;
@@ -1782,9 +1783,6 @@ void launch_copy_counter(hipStream_t s, const int *src, int *dst) {
 }
 void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_assemble<<<nb, kBlock, 0, s>>>(d, m);
-}
-void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
-  if (nb) k_bands_to_f32<<<nb, kBlock, 0, s>>>(d, m);
 }
 void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed) {
   if (!nb) return;

@@ -7,7 +7,6 @@ namespace sg {
 void launch_begin(hipStream_t s, const InstDev *d, int n, const int *part, int *active,
                   int *oactive, double tol2, double otol2, int maxit, int omaxit);
 void launch_assemble(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
-void launch_bands_to_f32(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_pweights(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_rap(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool mixed);
 void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l,
