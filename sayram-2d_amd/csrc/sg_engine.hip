@@ -905,7 +905,7 @@ int Batch::step_once() {
     tend(1);
   }
   if (!mixed) {
-    launch_bicg_init64(stream, d_insts, dm.cmap[0], nb0);
+    // the assembly started BiCGSTAB (r = rh = R - M f, p = v = 0, partials)
     launch_finish<double>(stream, FIN_INIT64, d_insts, n, d_cnt);
     if ((rc = h32 ? krylov<double, float>(opts.max_iters, true)
                   : krylov<double, double>(opts.max_iters, true)))

@@ -113,6 +113,16 @@ __device__ __forceinline__ void block_sum(A (&v)[NV]) {
   __syncthreads();
 }
 
+// partial sums of a workgroup (NV values) into slot lblk of the instance
+template <typename A, int NV>
+__device__ __forceinline__ void store_partials(const InstDev &I, int lblk, A (&v)[NV]) {
+  block_sum<A, NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) I.partial[(size_t)k * I.nblk0 + lblk] = (double)v[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // step start: statuses, tolerances and the active counters (a kernel rather
 // than a memset, so the sequence can be captured into a graph)
@@ -253,10 +263,28 @@ __device__ __forceinline__ void dirichlet_face(const CellEdges &c, int inbr, con
   AK = aA + aB + Bp / (fK + kEps);
 }
 
+// r = R - M x of the FP64 Krylov start, with the products summed as spmv64
+// sums them (centre, west, east, south, north; fused multiply-adds), from
+// the band values and x of the cell and its neighbours (couplings outside the
+// grid are zero).
+__device__ __forceinline__ double start_residual(double R, double ac, double x0, double aw,
+                                                 double xw, double ae, double xe, double as,
+                                                 double xs, double an, double xn) {
+#pragma clang fp contract(fast)
+  double s = ac * x0;
+  s += aw * xw;
+  s += ae * xe;
+  s += as * xs;
+  s += an * xn;
+  return R - s;
+}
+
 // One work-item per cell writes row K of M (d, w, e, s, n) and R. The diagonal
 // is summed in the order the reference pushes its triplets (west pairs, south
 // pairs, Dirichlet XMIN/XMAX/YMIN/YMAX, time term), which is the order Eigen's
-// setFromTriplets adds duplicates; R likewise. Also seeds x = f (warm start).
+// setFromTriplets adds duplicates; R likewise. Also seeds x = f (warm start)
+// and, for the FP64 Krylov forms, starts BiCGSTAB: r = rh = R - M x, p = v = 0
+// and the partial sums (r, r), (R, R).
 // Every array value the cell needs is loaded first, with neighbour indices
 // clamped into the grid (values of neighbours outside it are loaded but never
 // used), so that a work-item waits for memory once rather than behind each
@@ -270,98 +298,112 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
   if (rsc(I)->ostatus == ST_IDLE) return;
   const int nx = I.nx, ny = I.ny;
   const int K = m.k0 + threadIdx.x;
-  if (K >= nx * ny) return;
-  const int hx = half_x(nx);
-  int i, j;
-  cell_ij(K, nx, hx, i, j);
-  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-  const size_t r0 = (size_t)j * nx, rm = hs ? r0 - nx : r0, rp = hn ? r0 + nx : r0;
-  const int sm = xslot(hw ? i - 1 : i, hx), s0 = xslot(i, hx), sp = xslot(he ? i + 1 : i, hx);
-  const auto f = gbl(I.f);
-  const double f3[3][3] = {{f[rm + sm], f[rm + s0], f[rm + sp]},
-                           {f[r0 + sm], f[K], f[r0 + sp]},
-                           {f[rp + sm], f[rp + s0], f[rp + sp]}};
-  const auto Lxx = gbl(I.Lxx), Lxy = gbl(I.Lxy), Lyy = gbl(I.Lyy);
-  const size_t kw = r0 + sm, ke = r0 + sp, ks = rm + s0, kn = rp + s0;
-  const double LK[3] = {Lxx[K], Lxy[K], Lyy[K]};
-  const double LW[3] = {Lxx[kw], Lxy[kw], Lyy[kw]}, LE[3] = {Lxx[ke], Lxy[ke], Lyy[ke]};
-  const double LS[3] = {Lxx[ks], Lxy[ks], Lyy[ks]}, LN[3] = {Lxx[kn], Lxy[kn], Lyy[kn]};
-  const auto xe = gbl(I.xe), ye = gbl(I.ye);
-  const double xm = xe[hw ? i - 1 : i], x0 = xe[i], x1 = xe[i + 1], x2 = xe[he ? i + 2 : i + 1];
-  const double ym = ye[hs ? j - 1 : j], y0 = ye[j], y1 = ye[j + 1], y2 = ye[hn ? j + 2 : j + 1];
-  const auto wx0 = gbl(I.wx0), wx1 = gbl(I.wx1), wy0 = gbl(I.wy0), wy1 = gbl(I.wy1);
-  const double wxLi = wx0[i], wxRi = wx1[i], wxLe = wx0[i + 1], wxRe = wx1[i + 1];
-  const double wyBj = wy0[j], wyTj = wy1[j], wyBn = wy0[j + 1], wyTn = wy1[j + 1];
-  const double GK = gbl(I.G)[K], itK = gbl(I.itau)[K];
-  const CellEdges cK{x0, x1, y0, y1}, cW{xm, x0, y0, y1}, cE{x1, x2, y0, y1};
-  const CellEdges cS{x0, x1, ym, y0}, cN{x0, x1, y1, y2};
-  const double vSW = vertex_value(I, i, j, f3[0][0], f3[0][1], f3[1][0], f3[1][1], wxLi, wxRi,
-                                  wyBj, wyTj);
-  const double vSE = vertex_value(I, i + 1, j, f3[0][1], f3[0][2], f3[1][1], f3[1][2], wxLe, wxRe,
-                                  wyBj, wyTj);
-  const double vNW = vertex_value(I, i, j + 1, f3[1][0], f3[1][1], f3[2][0], f3[2][1], wxLi, wxRi,
-                                  wyBn, wyTn);
-  const double vNE = vertex_value(I, i + 1, j + 1, f3[1][1], f3[1][2], f3[2][1], f3[2][2], wxLe,
-                                  wxRe, wyBn, wyTn);
-  const double fK = f3[1][1];
-  double d = 0.0, R = 0.0, bw = 0.0, be = 0.0, bs = 0.0, bn = 0.0, AK, AL;
-  if (hw) {   // own west face (west-pair loop, :173-175)
-    face_pair(cK, cW, 0, LK, LW, fK, f3[1][0], vNW, vSW, AK, AL);
-    d += AK;
-    bw = 0.0 + -AL;
+  const bool krylov64 = I.prec != SG_MIXED;   // uniform over the workgroup
+  double part[2] = {0.0, 0.0};
+  if (K < nx * ny) {
+    const int hx = half_x(nx);
+    int i, j;
+    cell_ij(K, nx, hx, i, j);
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const size_t r0 = (size_t)j * nx, rm = hs ? r0 - nx : r0, rp = hn ? r0 + nx : r0;
+    const int sm = xslot(hw ? i - 1 : i, hx), s0 = xslot(i, hx), sp = xslot(he ? i + 1 : i, hx);
+    const auto f = gbl(I.f);
+    const double f3[3][3] = {{f[rm + sm], f[rm + s0], f[rm + sp]},
+                             {f[r0 + sm], f[K], f[r0 + sp]},
+                             {f[rp + sm], f[rp + s0], f[rp + sp]}};
+    const auto Lxx = gbl(I.Lxx), Lxy = gbl(I.Lxy), Lyy = gbl(I.Lyy);
+    const size_t kw = r0 + sm, ke = r0 + sp, ks = rm + s0, kn = rp + s0;
+    const double LK[3] = {Lxx[K], Lxy[K], Lyy[K]};
+    const double LW[3] = {Lxx[kw], Lxy[kw], Lyy[kw]}, LE[3] = {Lxx[ke], Lxy[ke], Lyy[ke]};
+    const double LS[3] = {Lxx[ks], Lxy[ks], Lyy[ks]}, LN[3] = {Lxx[kn], Lxy[kn], Lyy[kn]};
+    const auto xe = gbl(I.xe), ye = gbl(I.ye);
+    const double xm = xe[hw ? i - 1 : i], x0 = xe[i], x1 = xe[i + 1], x2 = xe[he ? i + 2 : i + 1];
+    const double ym = ye[hs ? j - 1 : j], y0 = ye[j], y1 = ye[j + 1], y2 = ye[hn ? j + 2 : j + 1];
+    const auto wx0 = gbl(I.wx0), wx1 = gbl(I.wx1), wy0 = gbl(I.wy0), wy1 = gbl(I.wy1);
+    const double wxLi = wx0[i], wxRi = wx1[i], wxLe = wx0[i + 1], wxRe = wx1[i + 1];
+    const double wyBj = wy0[j], wyTj = wy1[j], wyBn = wy0[j + 1], wyTn = wy1[j + 1];
+    const double GK = gbl(I.G)[K], itK = gbl(I.itau)[K];
+    const CellEdges cK{x0, x1, y0, y1}, cW{xm, x0, y0, y1}, cE{x1, x2, y0, y1};
+    const CellEdges cS{x0, x1, ym, y0}, cN{x0, x1, y1, y2};
+    const double vSW = vertex_value(I, i, j, f3[0][0], f3[0][1], f3[1][0], f3[1][1], wxLi, wxRi,
+                                    wyBj, wyTj);
+    const double vSE = vertex_value(I, i + 1, j, f3[0][1], f3[0][2], f3[1][1], f3[1][2], wxLe, wxRe,
+                                    wyBj, wyTj);
+    const double vNW = vertex_value(I, i, j + 1, f3[1][0], f3[1][1], f3[2][0], f3[2][1], wxLi, wxRi,
+                                    wyBn, wyTn);
+    const double vNE = vertex_value(I, i + 1, j + 1, f3[1][1], f3[1][2], f3[2][1], f3[2][2], wxLe,
+                                    wxRe, wyBn, wyTn);
+    const double fK = f3[1][1];
+    double d = 0.0, R = 0.0, bw = 0.0, be = 0.0, bs = 0.0, bn = 0.0, AK, AL;
+    if (hw) {   // own west face (west-pair loop, :173-175)
+      face_pair(cK, cW, 0, LK, LW, fK, f3[1][0], vNW, vSW, AK, AL);
+      d += AK;
+      bw = 0.0 + -AL;
+    }
+    if (he) {   // east face, owned by (i+1, j): this cell is its L
+      face_pair(cE, cK, 0, LE, LK, f3[1][2], fK, vNE, vSE, AK, AL);
+      d += AL;
+      be = 0.0 + -AK;
+    }
+    if (hs) {   // own south face (south-pair loop, :178-182)
+      face_pair(cK, cS, 3, LK, LS, fK, f3[0][1], vSW, vSE, AK, AL);
+      d += AK;
+      bs = 0.0 + -AL;
+    }
+    if (hn) {   // north face, owned by (i, j+1)
+      face_pair(cN, cK, 3, LN, LK, f3[2][1], fK, vNW, vNE, AK, AL);
+      d += AL;
+      bn = 0.0 + -AK;
+    }
+    double Bm;
+    if (i == 0 && (I.bcmask & 1)) {        // XMIN: face im
+      dirichlet_face(cK, 0, LK, fK, vNW, vSW, AK, Bm);
+      d += AK; R += Bm;
+    }
+    if (i == nx - 1 && (I.bcmask & 2)) {   // XMAX: face ip
+      dirichlet_face(cK, 2, LK, fK, vSE, vNE, AK, Bm);
+      d += AK; R += Bm;
+    }
+    if (j == 0 && (I.bcmask & 4)) {        // YMIN: face jm
+      dirichlet_face(cK, 3, LK, fK, vSW, vSE, AK, Bm);
+      d += AK; R += Bm;
+    }
+    if (j == ny - 1 && (I.bcmask & 8)) {   // YMAX: face jp
+      dirichlet_face(cK, 1, LK, fK, vNE, vNW, AK, Bm);
+      d += AK; R += Bm;
+    }
+    // time / loss diagonal (:190-199), cell_area_dt (source/Mesh.h:61-63)
+    const double dx = x1 - x0, dy = y1 - y0;
+    const double UKK = GK * (dx * dy / I.dt);
+    d += UKK * (1 + I.dt * itK);
+    R += UKK * fK;
+    I.band64[BC][K] = d;
+    I.band64[BW][K] = bw;
+    I.band64[BE][K] = be;
+    I.band64[BS][K] = bs;
+    I.band64[BN][K] = bn;
+    if (I.prec != SG_FP64) {   // FP32 copies for the FP32 V-cycle and Krylov forms
+      const Lev &L0 = I.lev[0];
+      ptr<float>(L0.band[BC])[K] = (float)d;
+      ptr<float>(L0.band[BW])[K] = (float)bw;
+      ptr<float>(L0.band[BE])[K] = (float)be;
+      ptr<float>(L0.band[BS])[K] = (float)bs;
+      ptr<float>(L0.band[BN])[K] = (float)bn;
+    }
+    I.R[K] = R;
+    I.x[K] = fK;
+    if (krylov64) {
+      const double r = start_residual(R, d, fK, bw, f3[1][0], be, f3[1][2], bs, f3[0][1], bn,
+                                      f3[2][1]);
+      gbl(ptr<double>(I.kv[VR]))[K] = r;
+      gbl(ptr<double>(I.kv[VRH]))[K] = r;
+      gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
+      gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
+      part[0] = r * r;
+      part[1] = R * R;
+    }
   }
-  if (he) {   // east face, owned by (i+1, j): this cell is its L
-    face_pair(cE, cK, 0, LE, LK, f3[1][2], fK, vNE, vSE, AK, AL);
-    d += AL;
-    be = 0.0 + -AK;
-  }
-  if (hs) {   // own south face (south-pair loop, :178-182)
-    face_pair(cK, cS, 3, LK, LS, fK, f3[0][1], vSW, vSE, AK, AL);
-    d += AK;
-    bs = 0.0 + -AL;
-  }
-  if (hn) {   // north face, owned by (i, j+1)
-    face_pair(cN, cK, 3, LN, LK, f3[2][1], fK, vNW, vNE, AK, AL);
-    d += AL;
-    bn = 0.0 + -AK;
-  }
-  double Bm;
-  if (i == 0 && (I.bcmask & 1)) {        // XMIN: face im
-    dirichlet_face(cK, 0, LK, fK, vNW, vSW, AK, Bm);
-    d += AK; R += Bm;
-  }
-  if (i == nx - 1 && (I.bcmask & 2)) {   // XMAX: face ip
-    dirichlet_face(cK, 2, LK, fK, vSE, vNE, AK, Bm);
-    d += AK; R += Bm;
-  }
-  if (j == 0 && (I.bcmask & 4)) {        // YMIN: face jm
-    dirichlet_face(cK, 3, LK, fK, vSW, vSE, AK, Bm);
-    d += AK; R += Bm;
-  }
-  if (j == ny - 1 && (I.bcmask & 8)) {   // YMAX: face jp
-    dirichlet_face(cK, 1, LK, fK, vNE, vNW, AK, Bm);
-    d += AK; R += Bm;
-  }
-  // time / loss diagonal (:190-199), cell_area_dt (source/Mesh.h:61-63)
-  const double dx = x1 - x0, dy = y1 - y0;
-  const double UKK = GK * (dx * dy / I.dt);
-  d += UKK * (1 + I.dt * itK);
-  R += UKK * fK;
-  I.band64[BC][K] = d;
-  I.band64[BW][K] = bw;
-  I.band64[BE][K] = be;
-  I.band64[BS][K] = bs;
-  I.band64[BN][K] = bn;
-  if (I.prec != SG_FP64) {   // FP32 copies for the FP32 V-cycle and Krylov forms
-    const Lev &L0 = I.lev[0];
-    ptr<float>(L0.band[BC])[K] = (float)d;
-    ptr<float>(L0.band[BW])[K] = (float)bw;
-    ptr<float>(L0.band[BE])[K] = (float)be;
-    ptr<float>(L0.band[BS])[K] = (float)bs;
-    ptr<float>(L0.band[BN])[K] = (float)bn;
-  }
-  I.R[K] = R;
-  I.x[K] = fK;
+  if (krylov64) store_partials<double, 2>(I, m.k0 / kBlock, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,38 +1370,6 @@ __device__ __forceinline__ T spmv5(const InstDev &I, size_t K, int i, int j, con
   return stencil_apply<T, 5>(I.lev[0], K, i, j, x);
 }
 
-template <typename A, int NV>
-__device__ __forceinline__ void store_partials(const InstDev &I, int lblk, A (&v)[NV]) {
-  block_sum<A, NV>(v);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) I.partial[(size_t)k * I.nblk0 + lblk] = (double)v[k];
-  }
-}
-
-// FP64 start: r = R - M x, rh = r, p = v = 0; partials (r, r), (R, R).
-__global__ void __launch_bounds__(kBlock) k_bicg_init64(const InstDev *__restrict__ insts,
-                                                       const BlkMap *__restrict__ map) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (rsc(I)->status != ST_ACTIVE) return;
-  const int K = m.k0 + threadIdx.x;
-  const int N = I.nx * I.ny;
-  double v[2] = {0.0, 0.0};
-  if (K < N) {
-    int i, j;
-    cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const double r = gbl(I.R)[K] - spmv64(I, K, i, j, I.x);   // the FP64 bands in every mode
-    gbl(ptr<double>(I.kv[VR]))[K] = r;
-    gbl(ptr<double>(I.kv[VRH]))[K] = r;
-    gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
-    gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
-    v[0] = r * r;
-    v[1] = gbl(I.R)[K] * gbl(I.R)[K];
-  }
-  store_partials<double, 2>(I, m.k0 / kBlock, v);
-}
-
 // PC (FP64 Krylov, FP32 V-cycle): the preconditioned vectors ph = M^-1 p and
 // sh = M^-1 s are FP32 V-cycle outputs scaled by pcs, the residual norm of
 // the iteration (the V-cycle input is the vector divided by pcs, so it stays
@@ -1890,9 +1900,6 @@ template void launch_prolong<double>(hipStream_t, const InstDev *, const BlkMap 
 template void launch_prolong<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
                                     int, int);
 
-void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
-  if (nb) k_bicg_init64<<<nb, kBlock, 0, s>>>(d, m);
-}
 template <typename T, bool PC>
 static void launch_bicg_t(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb,
                           bool xstate) {

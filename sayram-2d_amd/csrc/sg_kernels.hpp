@@ -32,7 +32,6 @@ void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, in
 template <typename T>
 void launch_coarse_cycle(hipStream_t s, const InstDev *d, const FuseMap *m, int n, int xid, int bid,
                          int smoother);
-void launch_bicg_init64(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 // which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update.
 // pc: FP64 Krylov vectors with FP32 preconditioned vectors (T must be double).
 template <typename T>
