@@ -504,6 +504,49 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
 // coarse point C = (I, J), whose support is the 3x3 fine block around (2I, 2J).
 // All offsets are compile-time constants (fine offset fa, fb; stencil offset;
 // parent slot q), so the 3x3 accumulator lives in registers.
+// The terms of fine row 2 J + fb of the support of coarse point (Ic, J):
+// acc[aj][ai] += P(f, C) A(f, g) P(g, C') for the fine points f of that row
+// inside the grid, their stencil neighbours g inside it and the parents C' of
+// g, in that order.
+template <typename T, int NB>
+__device__ __forceinline__ void rap_row(const Lev &F, const T *const (&pw)[4], int Ic, int J,
+                                        int fb, int hxf, double (&acc)[3][3]) {
+  constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
+  constexpr int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
+  const int fx = F.nx, fy = F.ny;
+  const int fj = 2 * J + fb;
+  if (fj < 0 || fj >= fy) return;
+#pragma unroll
+  for (int fa = -1; fa <= 1; ++fa) {
+    const int fi = 2 * Ic + fa;
+    if (fi < 0 || fi >= fx) continue;
+    const size_t kf = (size_t)fj * fx + xslot(fi, hxf);
+    // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
+    const int slot = (fa < 0 ? 1 : 0) + 2 * (fb < 0 ? 1 : 0);
+    const double wf = (double)gbl(pw[slot])[kf];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int ga = fa + di_[b], gb = fb + dj_[b];
+      const int gi = 2 * Ic + ga, gj = 2 * J + gb;
+      if (gi < 0 || gi >= fx || gj < 0 || gj >= fy) continue;
+      const double wa = wf * gbl(F.band64[b])[kf];
+      const size_t kg = (size_t)gj * fx + xslot(gi, hxf);
+      const int oi = (ga >= 0 ? ga / 2 : -((1 - ga) / 2)) + 1;   // floor(ga / 2) + 1
+      const int oj = (gb >= 0 ? gb / 2 : -((1 - gb) / 2)) + 1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ai = oi + (q & 1), aj = oj + (q >> 1);
+        if (ai > 2 || aj > 2) continue;   // such parents carry zero weight
+        // so do the parents at I0 + 1 of an even gi and at J0 + 1 of an even
+        // gj (gi, gj have the parities of ga, gb): k_pweights writes exact
+        // zeros there, and the sum skips them
+        if (((q & 1) && !(ga & 1)) || ((q & 2) && !(gb & 1))) continue;
+        acc[aj][ai] += wa * (double)gbl(pw[q])[kg];
+      }
+    }
+  }
+}
+
 template <typename T, int NB, bool COPY>
 __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ insts,
                                                const BlkMap *__restrict__ map, int l) {
@@ -512,7 +555,7 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   if (rsc(I)->ostatus == ST_IDLE) return;
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
-  const int cx = Cl.nx, cy = Cl.ny, fx = F.nx, fy = F.ny;
+  const int cx = Cl.nx, cy = Cl.ny, fx = F.nx;
   const int KN = m.k0 + threadIdx.x;
   if (KN >= cx * cy) return;
   const int hxf = half_x(fx);
@@ -522,41 +565,11 @@ __global__ void __launch_bounds__(kBlock) k_rap(const InstDev *__restrict__ inst
   // the point's slot.
   const int J = KN / cx, Ic = KN - J * cx;
   const size_t KC = (size_t)J * cx + xslot(Ic, half_x(cx));
-  const T *__restrict__ pw[4] = {cptr<T>(F.pw[0]), cptr<T>(F.pw[1]), cptr<T>(F.pw[2]),
-                                 cptr<T>(F.pw[3])};
-  constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
-  constexpr int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
+  const T *const pw[4] = {cptr<T>(F.pw[0]), cptr<T>(F.pw[1]), cptr<T>(F.pw[2]),
+                          cptr<T>(F.pw[3])};
   double acc[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
 #pragma unroll
-  for (int fb = -1; fb <= 1; ++fb) {
-    const int fj = 2 * J + fb;
-    if (fj < 0 || fj >= fy) continue;
-#pragma unroll
-    for (int fa = -1; fa <= 1; ++fa) {
-      const int fi = 2 * Ic + fa;
-      if (fi < 0 || fi >= fx) continue;
-      const size_t kf = (size_t)fj * fx + xslot(fi, hxf);
-      // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
-      const int slot = (fa < 0 ? 1 : 0) + 2 * (fb < 0 ? 1 : 0);
-      const double wf = (double)pw[slot][kf];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int ga = fa + di_[b], gb = fb + dj_[b];
-        const int gi = 2 * Ic + ga, gj = 2 * J + gb;
-        if (gi < 0 || gi >= fx || gj < 0 || gj >= fy) continue;
-        const double wa = wf * F.band64[b][kf];
-        const size_t kg = (size_t)gj * fx + xslot(gi, hxf);
-        const int oi = (ga >= 0 ? ga / 2 : -((1 - ga) / 2)) + 1;   // floor(ga / 2) + 1
-        const int oj = (gb >= 0 ? gb / 2 : -((1 - gb) / 2)) + 1;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int ai = oi + (q & 1), aj = oj + (q >> 1);
-          if (ai > 2 || aj > 2) continue;   // such parents carry zero weight
-          acc[aj][ai] += wa * (double)pw[q][kg];
-        }
-      }
-    }
-  }
+  for (int fb = -1; fb <= 1; ++fb) rap_row<T, NB>(F, pw, Ic, J, fb, hxf, acc);
   const double out[9] = {acc[1][1], acc[1][0], acc[1][2], acc[0][1], acc[2][1],
                          acc[0][0], acc[0][2], acc[2][0], acc[2][2]};
 #pragma unroll
