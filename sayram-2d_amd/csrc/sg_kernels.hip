@@ -1214,8 +1214,49 @@ struct FuseLds {
   T up[kFuseCells];    // upper / pivot
 };
 
-template <typename T, int NB, int DIR>
-__device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool zero, FuseLds<T> &S) {
+// FP32 levels are staged into LDS whole (b, the off-diagonal bands and the
+// line factors of both directions) once per level, so that the four sweeps
+// read no global memory; FP64 levels (twice the bytes) read them from global
+// memory in each sweep.
+enum FuseStageArr { FS_B, FS_W, FS_E, FS_S, FS_N, FS_SW, FS_SE, FS_NW, FS_NE, FS_I0, FS_U0, FS_I1,
+                    FS_U1, FS_COUNT };
+template <typename T> struct FuseStage { T a[FS_COUNT][kFuseCells]; };
+struct FuseStageNone { char a[1][1]; };   // FP64: no staging
+template <typename T> struct FuseStageFor { using type = FuseStage<T>; };
+template <> struct FuseStageFor<double> { using type = FuseStageNone; };
+template <typename T> constexpr bool kFuseStaged = sizeof(T) == 4;
+
+template <typename T, int NB>
+__device__ void fused_stage(const InstDev &I, int l, int bid, typename FuseStageFor<T>::type &G) {
+  const Lev &Lv = I.lev[l];
+  const int nc = Lv.nx * Lv.ny;
+  const auto b = gbl(l == 0 ? cptr<T>(vec(I, bid)) : cptr<T>(Lv.b));
+  for (int K = threadIdx.x; K < nc; K += blockDim.x) {
+    G.a[FS_B][K] = b[K];
+    G.a[FS_W][K] = gbl(cptr<T>(Lv.band[BW]))[K];
+    G.a[FS_E][K] = gbl(cptr<T>(Lv.band[BE]))[K];
+    G.a[FS_S][K] = gbl(cptr<T>(Lv.band[BS]))[K];
+    G.a[FS_N][K] = gbl(cptr<T>(Lv.band[BN]))[K];
+    if (NB == 9) {
+      G.a[FS_SW][K] = gbl(cptr<T>(Lv.band[BSW]))[K];
+      G.a[FS_SE][K] = gbl(cptr<T>(Lv.band[BSE]))[K];
+      G.a[FS_NW][K] = gbl(cptr<T>(Lv.band[BNW]))[K];
+      G.a[FS_NE][K] = gbl(cptr<T>(Lv.band[BNE]))[K];
+    }
+    G.a[FS_I0][K] = gbl(cptr<T>(Lv.linv[0]))[K];
+    G.a[FS_U0][K] = gbl(cptr<T>(Lv.lup[0]))[K];
+    G.a[FS_I1][K] = gbl(cptr<T>(Lv.linv[1]))[K];
+    G.a[FS_U1][K] = gbl(cptr<T>(Lv.lup[1]))[K];
+  }
+}
+
+// One zebra sweep (lines of one colour) of level l. STG: the level's inputs
+// are in G (LDS); otherwise they are read from global memory and the line
+// factors copied to S.lo, S.inv, S.up. Both read the same values and do the
+// same operations.
+template <typename T, int NB, int DIR, bool STG>
+__device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool zero, FuseLds<T> &S,
+                            typename FuseStageFor<T>::type &G) {
   const Lev &Lv = I.lev[l];
   const int nx = Lv.nx, ny = Lv.ny, hx = half_x(nx);
   const T *b = l == 0 ? cptr<T>(vec(I, bid)) : cptr<T>(Lv.b);
@@ -1231,6 +1272,15 @@ __device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool z
   }
   const T *inv = cptr<T>(Lv.linv[DIR]);
   const T *cu = cptr<T>(Lv.lup[DIR]);
+  // staged copies of the same arrays
+  constexpr int gLo = DIR == 0 ? FS_W : FS_S, gOm = DIR == 0 ? FS_S : FS_W;
+  constexpr int gOp = DIR == 0 ? FS_N : FS_E, gOmm = FS_SW, gOmp = DIR == 0 ? FS_SE : FS_NW;
+  constexpr int gOpm = DIR == 0 ? FS_NW : FS_SE, gOpp = FS_NE;
+  constexpr int gInv = DIR == 0 ? FS_I0 : FS_I1, gUp = DIR == 0 ? FS_U0 : FS_U1;
+  auto val = [&](const T *p, int g, uint32_t K) -> T {
+    if constexpr (STG) return G.a[g][K];
+    else return ld32(p, K);
+  };
   const int n = DIR == 0 ? nx : ny, nperp = DIR == 0 ? ny : nx;
   const int nl = (nperp - colour + 1) >> 1;   // lines colour, colour + 2, ..
   auto at = [&](int li, int t) -> uint32_t {
@@ -1239,29 +1289,37 @@ __device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool z
   for (int idx = threadIdx.x; idx < nl * n; idx += blockDim.x) {
     const int q = idx / n, t = idx - q * n, li = colour + 2 * q;
     const uint32_t K = at(li, t);
-    T r = ld32(b, K);
+    T r = val(b, FS_B, K);
     if (!zero) {
       if (li > 0) {
-        r -= ld32(om, K) * S.x[at(li - 1, t)];
+        r -= val(om, gOm, K) * S.x[at(li - 1, t)];
         if (NB == 9) {
-          if (t > 0) r -= ld32(omm, K) * S.x[at(li - 1, t - 1)];
-          if (t < n - 1) r -= ld32(omp, K) * S.x[at(li - 1, t + 1)];
+          if (t > 0) r -= val(omm, gOmm, K) * S.x[at(li - 1, t - 1)];
+          if (t < n - 1) r -= val(omp, gOmp, K) * S.x[at(li - 1, t + 1)];
         }
       }
       if (li < nperp - 1) {
-        r -= ld32(op, K) * S.x[at(li + 1, t)];
+        r -= val(op, gOp, K) * S.x[at(li + 1, t)];
         if (NB == 9) {
-          if (t > 0) r -= ld32(opm, K) * S.x[at(li + 1, t - 1)];
-          if (t < n - 1) r -= ld32(opp, K) * S.x[at(li + 1, t + 1)];
+          if (t > 0) r -= val(opm, gOpm, K) * S.x[at(li + 1, t - 1)];
+          if (t < n - 1) r -= val(opp, gOpp, K) * S.x[at(li + 1, t + 1)];
         }
       }
     }
     S.r[K] = r;
-    S.lo[K] = t > 0 ? ld32(lo, K) : T(0);
-    S.inv[K] = ld32(inv, K);
-    S.up[K] = ld32(cu, K);
+    if constexpr (!STG) {
+      S.lo[K] = t > 0 ? ld32(lo, K) : T(0);
+      S.inv[K] = ld32(inv, K);
+      S.up[K] = ld32(cu, K);
+    }
   }
   __syncthreads();
+  const T *s_lo, *s_inv, *s_up;
+  if constexpr (STG) {
+    s_lo = G.a[gLo]; s_inv = G.a[gInv]; s_up = G.a[gUp];
+  } else {
+    s_lo = S.lo; s_inv = S.inv; s_up = S.up;
+  }
   for (int q = threadIdx.x; q < nl; q += blockDim.x) {
     const int li = colour + 2 * q;
     // (unrolled so that the LDS reads of several steps, which do not depend
@@ -1270,14 +1328,15 @@ __device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool z
 #pragma unroll 8
     for (int t = 0; t < n; ++t) {   // forward: d_t = (r_t - lo_t d_{t-1}) / pivot_t
       const uint32_t K = at(li, t);
-      d = (S.r[K] - S.lo[K] * d) * S.inv[K];
+      const T lv = t > 0 ? s_lo[K] : T(0);
+      d = (S.r[K] - lv * d) * s_inv[K];
       S.r[K] = d;
     }
     T xn = T(0);
 #pragma unroll 8
     for (int t = n - 1; t >= 0; --t) {   // backward: x_t = d_t - u_t x_{t+1}
       const uint32_t K = at(li, t);
-      xn = S.r[K] - S.up[K] * xn;
+      xn = S.r[K] - s_up[K] * xn;
       S.x[K] = xn;
     }
   }
@@ -1299,15 +1358,17 @@ __device__ void fused_x(const InstDev &I, int l, int xid, FuseLds<T> &S, bool lo
 
 template <typename T, int NB>
 __device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool post_x, bool post_y,
-                             FuseLds<T> &S) {
+                             FuseLds<T> &S, typename FuseStageFor<T>::type &G) {
+  constexpr bool STG = kFuseStaged<T>;
+  if constexpr (STG) fused_stage<T, NB>(I, l, bid, G);   // the barrier of fused_x covers it
   fused_x<T>(I, l, xid, S, true);
   if (post_x) {
-    fused_sweep<T, NB, 0>(I, l, bid, 0, false, S);
-    fused_sweep<T, NB, 0>(I, l, bid, 1, false, S);
+    fused_sweep<T, NB, 0, STG>(I, l, bid, 0, false, S, G);
+    fused_sweep<T, NB, 0, STG>(I, l, bid, 1, false, S, G);
   }
   if (post_y) {
-    fused_sweep<T, NB, 1>(I, l, bid, 0, false, S);
-    fused_sweep<T, NB, 1>(I, l, bid, 1, false, S);
+    fused_sweep<T, NB, 1, STG>(I, l, bid, 0, false, S, G);
+    fused_sweep<T, NB, 1, STG>(I, l, bid, 1, false, S, G);
   }
   fused_x<T>(I, l, xid, S, false);
 }
@@ -1317,6 +1378,7 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
                                                                const FuseMap *__restrict__ map,
                                                                int xid, int bid, int smoother) {
   __shared__ FuseLds<T> S;
+  __shared__ typename FuseStageFor<T>::type G;
   const FuseMap fm = map[blockIdx.x];
   const InstDev &I = insts[fm.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
@@ -1331,11 +1393,11 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   {   // exact solve of the coarsest level (x = 0): one row or one column
     const int c = nl - 1;
     if (I.lev[c].ny == 1) {
-      if (c == 0) fused_sweep<T, 5, 0>(I, c, bid, 0, true, S);
-      else fused_sweep<T, 9, 0>(I, c, bid, 0, true, S);
+      if (c == 0) fused_sweep<T, 5, 0, false>(I, c, bid, 0, true, S, G);
+      else fused_sweep<T, 9, 0, false>(I, c, bid, 0, true, S, G);
     } else {
-      if (c == 0) fused_sweep<T, 5, 1>(I, c, bid, 0, true, S);
-      else fused_sweep<T, 9, 1>(I, c, bid, 0, true, S);
+      if (c == 0) fused_sweep<T, 5, 1, false>(I, c, bid, 0, true, S, G);
+      else fused_sweep<T, 9, 1, false>(I, c, bid, 0, true, S, G);
     }
     fused_x<T>(I, c, xid, S, false);
   }
@@ -1352,8 +1414,8 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
       for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 0>(I, m, xid, k);
     }
     __syncthreads();
-    if (m == 0) fused_smooth<T, 5>(I, m, xid, bid, post_x, post_y, S);
-    else fused_smooth<T, 9>(I, m, xid, bid, post_x, post_y, S);
+    if (m == 0) fused_smooth<T, 5>(I, m, xid, bid, post_x, post_y, S, G);
+    else fused_smooth<T, 9>(I, m, xid, bid, post_x, post_y, S, G);
   }
 }
 
