@@ -81,12 +81,13 @@ def cpu_model():
 def run_batched(probs, warmup, steps, precision=sg.SG_MIXED, graph=True, time_kernel=True,
                 opts=None, streams=0):
     """Batched steps. time_kernel brackets every level-0 sweep dispatch with
-    HIP events; such a run launches eagerly and on one stream (streams=1), so
-    that each dispatch's interval is that kernel alone."""
+    HIP events; such a run launches eagerly, on one stream and in one instance
+    group (streams=1, group_cells=-1), so that each dispatch's interval is
+    that kernel alone."""
     if opts is None:
         opts = sg.make_opts(precision=precision, use_graph=1 if graph else -1, streams=streams)
     if time_kernel:
-        opts.use_graph, opts.streams = -1, 1
+        opts.use_graph, opts.streams, opts.group_cells = -1, 1, -1
     b = sg.Batch(opts)
     slots = [b.add(p) for p in probs]
     b.step(warmup)
@@ -299,8 +300,9 @@ def main():
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                            "avg_launch_ms": avg_ms, "launches": kl,
                            "timing": "HIP events around every dispatch on its stream, in an "
-                                     "eager run on one stream so that each interval is the "
-                                     "kernel alone (the headline run overlaps tile shapes)",
+                                     "eager run on one stream and one instance group so that each "
+                                     "interval is the kernel alone (the headline run overlaps tile "
+                                     "shapes and instance groups)",
                            "algorithmic_bytes_per_launch": kbytes / kl}
         # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
         # runs recorded beside the kernel-trace summary

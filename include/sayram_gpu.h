@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 1
+#define SG_ABI_VERSION 2
 
 typedef enum {
   SG_OK = 0,
@@ -105,6 +105,13 @@ typedef struct {
   int streams;          /* 2: the tile shapes of one line sweep run on two HIP
                            streams (the longest lines beside the rest); 1: one
                            stream. 0 selects 2. Results are identical. */
+  int group_cells;      /* a batch steps its instances of at least group_cells
+                           cells as a second group, on its own stream and host
+                           thread, concurrently with the rest (their long line
+                           solves are latency-bound, the others' work
+                           bandwidth-bound). 0 selects 2^21; -1 keeps one group.
+                           Results are identical. */
+  int reserved[1];
 } sg_opts;
 
 typedef struct {

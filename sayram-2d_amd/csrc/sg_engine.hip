@@ -13,6 +13,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace sg {
@@ -292,6 +293,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 2;
+  if (o.group_cells == 0) o.group_cells = 1 << 21;
   const int tiles[4] = {24, 160, 24, 384};
   for (int k = 0; k < 4; ++k)
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
@@ -1059,7 +1061,76 @@ int Batch::debug_assemble(int slot, double *bands, double *R) {
 using sg::Batch;
 using sg::fail;
 
-struct sg_batch { Batch b; };
+// A batch keeps its instances in two groups: instances of at least
+// opts.group_cells cells (whose long line solves are latency-bound) and the
+// rest (bandwidth-bound). Each group is an engine Batch with its own stream;
+// a step advances the two concurrently, the large group on a second host
+// thread. Instances are independent, so results do not depend on the
+// grouping. Slots seen by the caller are global; `where` maps them to a group
+// and that group's slot.
+struct sg_batch {
+  Batch g[2];
+  bool inited[2] = {false, false};
+  sg_opts opts{};
+  size_t budget = 0;
+  int device = 0;
+  std::vector<std::pair<int, int>> where;   // global slot -> (group, local slot); group -1: free
+  int time_id = -1;
+
+  Batch *group_of(int slot, int *local) {
+    if (slot < 0 || slot >= (int)where.size() || where[slot].first < 0) return nullptr;
+    *local = where[slot].second;
+    return &g[where[slot].first];
+  }
+  const Batch *group_of(int slot, int *local) const {
+    return const_cast<sg_batch *>(this)->group_of(slot, local);
+  }
+  int ensure_group(int k) {
+    if (inited[k]) return SG_OK;
+    int rc = g[k].init(&opts, 0);
+    if (rc) return rc;
+    g[k].time_id = time_id;
+    inited[k] = true;
+    return SG_OK;
+  }
+  size_t in_use() const { return g[0].in_use + g[1].in_use; }
+  int size() const {
+    int n = 0;
+    for (auto &w : where) n += w.first >= 0;
+    return n;
+  }
+  // Steps both groups (targets indexed by global slot, as sg_batch_step_until).
+  int step(int nsteps, const int64_t *targets, int nslots) {
+    std::vector<int64_t> t[2];
+    if (targets) {
+      for (int k = 0; k < 2; ++k) t[k].assign(g[k].slots.size(), -1);
+      for (int s = 0; s < (int)where.size() && s < nslots; ++s)
+        if (where[s].first >= 0) t[where[s].first][where[s].second] = targets[s];
+    }
+    auto run = [&](int k) {
+      return g[k].step(nsteps, targets ? t[k].data() : nullptr, targets ? (int)t[k].size() : 0);
+    };
+    const bool busy[2] = {inited[0] && has(0), inited[1] && has(1)};
+    if (!(busy[0] && busy[1])) return busy[1] ? run(1) : busy[0] ? run(0) : SG_OK;
+    int rc1 = SG_OK;
+    std::string err1;
+    std::thread worker([&] {
+      (void)hipSetDevice(device);
+      rc1 = run(1);
+      if (rc1) err1 = sg::g_err;
+    });
+    const int rc0 = run(0);
+    worker.join();
+    if (rc0 && rc0 != SG_ERR_NOCONV) return rc0;
+    if (rc1) { sg::g_err = err1; return rc1; }
+    return rc0;
+  }
+  bool has(int k) const {
+    for (auto &w : where)
+      if (w.first == k) return true;
+    return false;
+  }
+};
 struct sg_solver { sg_batch *bt; int slot; };
 
 extern "C" {
@@ -1083,8 +1154,13 @@ int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
   if (!out) return fail(SG_ERR_ARG, "out must not be null");
   *out = nullptr;
   std::unique_ptr<sg_batch> b(new sg_batch);
-  int rc = b->b.init(opts, budget);
+  if (opts) b->opts = *opts;
+  b->budget = budget;
+  // group 0 is created here, so option errors and a missing GPU show at once
+  int rc = b->ensure_group(0);
   if (rc) return rc;
+  b->opts = b->g[0].opts;   // with the defaults applied
+  (void)hipGetDevice(&b->device);
   *out = b.release();
   return SG_OK;
 }
@@ -1092,68 +1168,99 @@ int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
 int sg_batch_add(sg_batch *b, const sg_grid *g, const sg_coeffs *c, const sg_bc *bc,
                  const double *f0, double dt, int *slot) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  return b->b.add(g, c, bc, f0, dt, slot);
+  if (!g) return fail(SG_ERR_ARG, "grid must not be null");
+  const long long cells = (long long)g->nx * g->ny;
+  const int k = (b->opts.group_cells > 0 && cells >= b->opts.group_cells) ? 1 : 0;
+  int rc = b->ensure_group(k);
+  if (rc) return rc;
+  // one HBM budget for both groups
+  Batch &G = b->g[k];
+  const size_t other = b->g[1 - k].in_use;
+  G.budget = b->budget ? (b->budget > other ? b->budget - other : 1) : 0;
+  int local = -1;
+  if ((rc = G.add(g, c, bc, f0, dt, &local))) return rc;
+  int gs = -1;
+  for (int s = 0; s < (int)b->where.size(); ++s)
+    if (b->where[s].first < 0) { gs = s; break; }
+  if (gs < 0) { gs = (int)b->where.size(); b->where.push_back({-1, -1}); }
+  b->where[gs] = {k, local};
+  if (slot) *slot = gs;
+  return SG_OK;
 }
 int sg_batch_remove(sg_batch *b, int slot) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  return b->b.remove(slot);
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  int rc = G->remove(local);
+  if (rc) return rc;
+  b->where[slot] = {-1, -1};
+  return SG_OK;
 }
 int sg_batch_step(sg_batch *b, int nsteps) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  return b->b.step(nsteps, nullptr, 0);
+  if (nsteps < 0) return fail(SG_ERR_ARG, "nsteps must be >= 0");
+  return b->step(nsteps, nullptr, 0);
 }
 int sg_batch_step_until(sg_batch *b, const int64_t *targets, int nslots, int nsteps) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
   if (!targets) return fail(SG_ERR_ARG, "targets must not be null");
-  return b->b.step(nsteps, targets, nslots);
+  if (nsteps < 0) return fail(SG_ERR_ARG, "nsteps must be >= 0");
+  return b->step(nsteps, targets, nslots);
 }
 int sg_batch_get_f(sg_batch *b, int slot, double *out) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  return b->b.get_f(slot, out);
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  return G->get_f(local, out);
 }
 int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  return b->b.set_f(slot, f, istep);
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  return G->set_f(local, f, istep);
 }
 int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  if (slot < 0 || slot >= (int)b->b.slots.size() || !b->b.slots[slot].used)
-    return fail(SG_ERR_ARG, "no instance in slot");
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot");
   if (side < 0 || side > 3) return fail(SG_ERR_ARG, "side must be 0..3");
-  auto &s = b->b.slots[slot];
+  auto &s = G->slots[local];
   if (!(s.dev.bcmask & (1 << side))) return fail(SG_ERR_ARG, "side is not Dirichlet");
-  return b->b.upload_bc(s, side, values);
+  return G->upload_bc(s, side, values);
 }
 int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out) {
   if (!b || !out) return fail(SG_ERR_ARG, "null argument");
-  if (slot < 0 || slot >= (int)b->b.slots.size() || !b->b.slots[slot].used)
-    return fail(SG_ERR_ARG, "no instance in slot");
-  *out = b->b.slots[slot].st;
+  int local;
+  const Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot");
+  *out = G->slots[local].st;
   return SG_OK;
 }
-int sg_batch_size(const sg_batch *b) {
-  if (!b) return 0;
-  int n = 0;
-  for (auto &s : b->b.slots) n += s.used;
-  return n;
-}
-size_t sg_batch_bytes_in_use(const sg_batch *b) { return b ? b->b.in_use : 0; }
+int sg_batch_size(const sg_batch *b) { return b ? b->size() : 0; }
+size_t sg_batch_bytes_in_use(const sg_batch *b) { return b ? b->in_use() : 0; }
 void sg_batch_destroy(sg_batch *b) { delete b; }
 
 int sg_batch_time_kernel(sg_batch *b, int kernel_id) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
   if (kernel_id >= sg::kNumKernelIds) return fail(SG_ERR_ARG, "unknown kernel id");
-  b->b.time_id = kernel_id;
-  b->b.t_ms = 0;
-  b->b.t_launches = 0;
-  b->b.t_bytes = 0;
+  b->time_id = kernel_id;
+  for (auto &G : b->g) {
+    G.time_id = kernel_id;
+    G.t_ms = 0;
+    G.t_launches = 0;
+    G.t_bytes = 0;
+  }
   return SG_OK;
 }
 int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  if (total_ms) *total_ms = b->b.t_ms;
-  if (launches) *launches = b->b.t_launches;
-  if (bytes) *bytes = b->b.t_bytes;
+  if (total_ms) *total_ms = b->g[0].t_ms + b->g[1].t_ms;
+  if (launches) *launches = b->g[0].t_launches + b->g[1].t_launches;
+  if (bytes) *bytes = b->g[0].t_bytes + b->g[1].t_bytes;
   return SG_OK;
 }
 const char *sg_kernel_name(int id) {
@@ -1168,7 +1275,7 @@ int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, con
   int rc = sg_batch_create(opts, 0, &b);
   if (rc) return rc;
   int slot = -1;
-  rc = b->b.add(grid, coeffs, bc, f0, dt, &slot);
+  rc = sg_batch_add(b, grid, coeffs, bc, f0, dt, &slot);
   if (rc) { delete b; return rc; }
   *out = new sg_solver{b, slot};
   return SG_OK;
@@ -1181,23 +1288,29 @@ int sg_set_coeffs(sg_solver *h, const sg_coeffs *c) {
   if (!h) return fail(SG_ERR_ARG, "null handle");
   int rc = sg::check_coeffs(c);
   if (rc) return rc;
-  return h->bt->b.upload_coeffs(h->bt->b.slots[h->slot], c);
+  int local;
+  Batch *G = h->bt->group_of(h->slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance");
+  return G->upload_coeffs(G->slots[local], c);
 }
 int sg_step(sg_solver *h, int nsteps) {
   if (!h) return fail(SG_ERR_ARG, "null handle");
-  return h->bt->b.step(nsteps, nullptr, 0);
+  return sg_batch_step(h->bt, nsteps);
 }
 int sg_get_f(sg_solver *h, double *out) {
   if (!h) return fail(SG_ERR_ARG, "null handle");
-  return h->bt->b.get_f(h->slot, out);
+  return sg_batch_get_f(h->bt, h->slot, out);
 }
 int sg_set_f(sg_solver *h, const double *f, int64_t istep) {
   if (!h) return fail(SG_ERR_ARG, "null handle");
-  return h->bt->b.set_f(h->slot, f, istep);
+  return sg_batch_set_f(h->bt, h->slot, f, istep);
 }
 double sg_time(const sg_solver *h) {
   if (!h) return 0.0;
-  const auto &s = h->bt->b.slots[h->slot];
+  int local;
+  const Batch *G = h->bt->group_of(h->slot, &local);
+  if (!G) return 0.0;
+  const auto &s = G->slots[local];
   return (double)s.steps * s.dt;   // Solver::t(), source/Solver.h:23
 }
 int sg_stats(const sg_solver *h, sg_step_stats *out) {
@@ -1211,7 +1324,10 @@ void sg_destroy(sg_solver *h) {
 }
 int sg_debug_assemble(sg_solver *h, double *bands5, double *R) {
   if (!h || !bands5 || !R) return fail(SG_ERR_ARG, "null argument");
-  return h->bt->b.debug_assemble(h->slot, bands5, R);
+  int local;
+  Batch *G = h->bt->group_of(h->slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance");
+  return G->debug_assemble(local, bands5, R);
 }
 int sg_solver_batch(sg_solver *h, sg_batch **out) {
   if (!h || !out) return fail(SG_ERR_ARG, "null argument");

@@ -40,7 +40,8 @@ class SgOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("rtol", C.c_double), ("max_iters", C.c_int),
                 ("inner_rtol", C.c_double), ("max_inner", C.c_int), ("check_every", C.c_int),
                 ("smoother", C.c_int), ("use_graph", C.c_int), ("line_tiles", C.c_int * 4),
-                ("fuse_cells", C.c_int), ("streams", C.c_int)]
+                ("fuse_cells", C.c_int), ("streams", C.c_int),
+                ("group_cells", C.c_int), ("reserved", C.c_int * 1)]
 
 
 class SgStats(C.Structure):
@@ -132,7 +133,7 @@ def _p(a):
 
 def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inner=0,
               check_every=0, smoother=0, use_graph=1, line_tiles=(0, 0, 0, 0), fuse_cells=0,
-              streams=0):
+              streams=0, group_cells=0):
     o = SgOpts()
     o.precision, o.rtol, o.max_iters = precision, rtol, max_iters
     o.inner_rtol, o.max_inner, o.check_every = inner_rtol, max_inner, check_every
@@ -141,6 +142,7 @@ def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inne
         o.line_tiles[k] = int(line_tiles[k])
     o.fuse_cells = int(fuse_cells)
     o.streams = int(streams)
+    o.group_cells = int(group_cells)
     return o
 
 

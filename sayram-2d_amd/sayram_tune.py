@@ -12,10 +12,12 @@ that select how the step is launched, and returns the fastest set:
 * ``line_tiles``: the line lengths at which the line solves switch tile shape
   (rows: short / medium limits; columns: short / medium limits).
 * ``streams``: the tile shapes of a sweep on two HIP streams (2) or one (1).
+* ``group_cells``: the size from which instances are stepped as a second
+  group, concurrently with the rest (-1: one group).
 * ``fuse_cells``: the size below which an instance's coarse levels run in the
   fused one-workgroup cycle (-1: never).
 
-``check_every``, ``use_graph`` and ``streams`` never change results: every
+``check_every``, ``use_graph``, ``streams`` and ``group_cells`` never change results: every
 instance's result is bit-identical for all their values. ``line_tiles`` and
 ``fuse_cells`` change the rounding of the line solves (not their accuracy), so
 the tuned values are part of the options a run is reproduced with.
@@ -31,7 +33,8 @@ import time
 import sayram_gpu as sg
 
 DEFAULTS = {"check_every": 1, "use_graph": 1, "row_short": 24, "row_mid": 160,
-            "col_short": 24, "col_mid": 384, "streams": 2, "fuse_cells": 1024}
+            "col_short": 24, "col_mid": 384, "streams": 2, "fuse_cells": 1024,
+            "group_cells": 1 << 21}
 
 CANDIDATES = {
     "check_every": [1, 2, 3, 4],
@@ -42,13 +45,15 @@ CANDIDATES = {
     "col_short": [16, 24, 32],
     "streams": [1, 2],
     "fuse_cells": [-1, 256, 1024],
+    "group_cells": [-1, 1 << 18, 1 << 21],
 }
 
 
 def opts_for(precision: int, p: dict):
     return sg.make_opts(precision=precision, check_every=p["check_every"], use_graph=p["use_graph"],
                         line_tiles=(p["row_short"], p["row_mid"], p["col_short"], p["col_mid"]),
-                        streams=p.get("streams", 2), fuse_cells=p.get("fuse_cells", 1024))
+                        streams=p.get("streams", 2), fuse_cells=p.get("fuse_cells", 1024),
+                        group_cells=p.get("group_cells", 1 << 21))
 
 
 def measure(problems, precision: int, p: dict, steps: int = 2, warmup: int = 1) -> float:
@@ -67,8 +72,8 @@ def measure(problems, precision: int, p: dict, steps: int = 2, warmup: int = 1) 
 
 
 def tune(problems, precision: int = sg.SG_PC32, steps: int = 2, warmup: int = 1,
-         order=("check_every", "use_graph", "streams", "fuse_cells", "col_mid", "row_mid",
-                "col_short", "row_short"),
+         order=("check_every", "use_graph", "streams", "group_cells", "fuse_cells", "col_mid",
+                "row_mid", "col_short", "row_short"),
          log=None):
     """Coordinate descent from DEFAULTS. Returns (best parameters, best value,
     list of (parameters, cell-steps/s) for every measured point)."""

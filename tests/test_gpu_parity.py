@@ -159,3 +159,31 @@ def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph):
         b.close()
     for k in range(len(probs)):
         assert np.array_equal(out[1][k], out[2][k])
+
+
+def test_two_groups_equal_one_group_bitwise(gpu_lib):
+    """A batch steps its instances of at least group_cells cells as a second
+    group, concurrently with the rest. Grouping changes no result: slots,
+    per-instance targets, removal and re-use of a slot and the statistics
+    behave as with one group, and every state is equal bit for bit."""
+    shapes = [(40, 30), (120, 90), (64, 64), (200, 150), (33, 17)]
+    probs = [sc.seeded_case(nx, ny, seed=140 + k) for k, (nx, ny) in enumerate(shapes)]
+    extra = sc.seeded_case(150, 130, seed=150)
+    out = {}
+    for gc in (-1, 5000):
+        b = gpu_lib.Batch(gpu_lib.make_opts(precision=gpu_lib.SG_PC32, group_cells=gc))
+        slots = [b.add(p) for p in probs]
+        b.step(2)
+        b.remove(slots[1])
+        s_new = b.add(extra)
+        assert s_new == slots[1]
+        targets = [4, 3, 5, 3, 4]
+        b.step_until(targets, 3)
+        res = [b.f(s) for s in slots]
+        st = [b.stats(s) for s in slots]
+        out[gc] = (res, [(x["steps"], x["last_iters"], x["total_iters"]) for x in st])
+        b.close()
+    for k in range(len(probs)):
+        assert np.array_equal(out[-1][0][k], out[5000][0][k])
+    assert out[-1][1] == out[5000][1]
+    assert [x[0] for x in out[5000][1]] == [4, 3, 5, 3, 4]
