@@ -93,8 +93,8 @@ typedef struct {
   int line_tiles[4];    /* line-solve tile shapes by line length: rows of at most
                            [0] cells use 64x16 tiles, of at most [1] cells 16x64,
                            longer rows 4x256; columns likewise with [2] and [3],
-                           longer columns 16x256. 0 selects the defaults 24, 160,
-                           24, 384. They change the rounding of the line solves,
+                           longer columns 16x256. 0 selects the defaults 16, 160,
+                           16, 768. They change the rounding of the line solves,
                            so results are reproducible for the same values. */
   int fuse_cells;       /* the V-cycle levels of an instance from its first level
                            of at most fuse_cells cells down run in one fused

@@ -294,7 +294,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 2;
   if (o.group_cells == 0) o.group_cells = 1 << 21;
-  const int tiles[4] = {24, 160, 24, 384};
+  const int tiles[4] = {16, 160, 16, 768};
   for (int k = 0; k < 4; ++k)
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
 }

@@ -32,15 +32,15 @@ import time
 
 import sayram_gpu as sg
 
-DEFAULTS = {"check_every": 1, "use_graph": 1, "row_short": 24, "row_mid": 160,
-            "col_short": 24, "col_mid": 384, "streams": 2, "fuse_cells": 1024,
+DEFAULTS = {"check_every": 1, "use_graph": 1, "row_short": 16, "row_mid": 160,
+            "col_short": 16, "col_mid": 768, "streams": 2, "fuse_cells": 1024,
             "group_cells": 1 << 21}
 
 CANDIDATES = {
     "check_every": [1, 2, 3, 4],
     "use_graph": [1, -1],
     "row_mid": [96, 160, 256],
-    "col_mid": [256, 384, 768],
+    "col_mid": [384, 768, 1536],
     "row_short": [16, 24, 32],
     "col_short": [16, 24, 32],
     "streams": [1, 2],

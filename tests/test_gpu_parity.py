@@ -146,10 +146,10 @@ def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
 def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph):
     """The tile shapes of a sweep (and of the per-step factorisations) run on
     two streams by default; they solve disjoint lines, so every result equals
-    the one-stream schedule bit for bit. Line lengths 20, 100, 300 and 500 give
+    the one-stream schedule bit for bit. Line lengths 12, 100, 300 and 900 give
     sweeps with three tile shapes in both directions."""
     probs = [sc.seeded_case(nx, ny, seed=120 + k) for k, (nx, ny) in
-             enumerate([(20, 300), (100, 20), (300, 100), (500, 40), (60, 500)])]
+             enumerate([(12, 300), (100, 12), (300, 100), (900, 40), (60, 900)])]
     out = {}
     for ns in (1, 2):
         b = gpu_lib.Batch(gpu_lib.make_opts(precision=gpu_lib.SG_PC32, streams=ns, use_graph=graph))

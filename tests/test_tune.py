@@ -12,7 +12,7 @@ from conftest import rel_l2
 
 def test_tuner_defaults_are_the_library_defaults():
     o = tu.opts_for(sg.SG_PC32, tu.DEFAULTS)
-    assert list(o.line_tiles) == [24, 160, 24, 384]
+    assert list(o.line_tiles) == [16, 160, 16, 768]
     assert o.check_every == 1 and o.use_graph == 1 and o.streams == 2 and o.fuse_cells == 1024
     assert o.group_cells == 1 << 21
     for name, cands in tu.CANDIDATES.items():
