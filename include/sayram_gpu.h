@@ -92,8 +92,8 @@ typedef struct {
                            replayed; -1: launch every kernel eagerly */
   int line_tiles[4];    /* line-solve tile shapes by line length: rows of at most
                            [0] cells use 64x16 tiles, of at most [1] cells 16x64,
-                           longer rows 4x256; columns likewise with [2] and [3],
-                           longer columns 16x256. 0 selects the defaults 16, 160,
+                           longer rows 4x256; columns of at most [2] cells 64x16,
+                           of at most [3] cells 32x32, longer columns 16x256. 0 selects the defaults 16, 160,
                            16, 768. They change the rounding of the line solves,
                            so results are reproducible for the same values. */
   int fuse_cells;       /* the V-cycle levels of an instance from its first level

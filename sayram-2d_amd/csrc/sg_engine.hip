@@ -142,17 +142,20 @@ struct Maps {
 };
 
 // Tile shape of a line solve by line length and direction: short lines in
-// 64-line x 16-step tiles, longer ones in 16 x 64, long rows in 4 x 256 (rows
-// stay coalesced along the line), long columns in 16 x 256 with 1024 threads
-// (columns keep 16 lines per tile so that the accesses across the lines stay
-// coalesced). FP64 V-cycles keep the 9-point levels on 256-thread tiles,
-// whose loads fit the registers of 1024-thread tiles only in FP32. It depends
-// only on the instance, its level and the batch's precision, so results do
-// not depend on the rest of the batch.
+// 64-line x 16-step tiles, longer rows in 16 x 64, longer columns in 32 x 32
+// (32 columns of one colour read 128 contiguous bytes per row), long rows in
+// 4 x 256 (rows stay coalesced along the line), long columns in 16 x 256 with
+// 1024 threads (16 columns per tile keep the accesses across the lines
+// coalesced with 4x fewer serial tiles). FP64 V-cycles keep the long columns
+// of the 9-point levels on 16 x 64 tiles, whose loads fit the registers of
+// 1024-thread tiles only in FP32. It depends only on the instance, its level
+// and the batch's precision, so results do not depend on the rest of the
+// batch.
 static int tile_shape(const int (&lim)[4], int n, int dir, int l, bool fp64_vcycle) {
   if (n <= lim[2 * dir]) return 0;
   if (dir == 0) return n <= lim[1] ? 1 : 2;
-  return (n <= lim[3] || (fp64_vcycle && l > 0)) ? 1 : 3;
+  if (n <= lim[3]) return 4;
+  return (fp64_vcycle && l > 0) ? 1 : 3;
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
@@ -724,14 +727,17 @@ int Batch::collect_time() {
 }
 
 // One launch per tile shape with work (launch(stream, shape)). With a side
-// stream and two or more shapes, the last shape (the longest lines: its groups
+// stream and two or more shapes, the shape of the longest lines (its groups
 // are the critical path) runs on `side`, forked from and joined back into
 // `stream`; the shapes write disjoint lines, so the results do not change.
 // Under stream capture the fork and join become graph edges.
 template <typename F> void Batch::fan_out(const int (&cnt)[kNumTileShapes], F &&launch) {
   int nshape = 0, last = -1;
   for (int sh = 0; sh < kNumTileShapes; ++sh)
-    if (cnt[sh]) { ++nshape; last = sh; }
+    if (cnt[sh]) {
+      ++nshape;
+      if (last < 0 || shape_rank(sh) >= shape_rank(last)) last = sh;
+    }
   if (nshape < 2 || !side) {
     for (int sh = 0; sh < kNumTileShapes; ++sh)
       if (cnt[sh]) launch(stream, sh);
@@ -742,8 +748,8 @@ template <typename F> void Batch::fan_out(const int (&cnt)[kNumTileShapes], F &&
   note(hipStreamWaitEvent(side, ev_fork, 0));
   launch(side, last);
   note(hipEventRecord(ev_join, side));
-  for (int sh = 0; sh < last; ++sh)
-    if (cnt[sh]) launch(stream, sh);
+  for (int sh = 0; sh < kNumTileShapes; ++sh)
+    if (cnt[sh] && sh != last) launch(stream, sh);
   note(hipStreamWaitEvent(stream, ev_join, 0));
 }
 

@@ -30,16 +30,24 @@ constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
 // workgroup of NT threads, each thread owning 4 consecutive steps of a line in
 // the solve:
 //   0: 64 x 16 (256 threads)  short lines
-//   1: 16 x 64 (256 threads)  medium lines, and columns
+//   1: 16 x 64 (256 threads)  medium rows; long columns of FP64 9-point levels
 //   2:  4 x 256 (256 threads) long rows (a row is contiguous in memory)
 //   3: 16 x 256 (1024 threads) long columns (16 columns of one colour are 64
 //      contiguous bytes per row; 4x fewer serial tiles than shape 1)
-constexpr int kNumTileShapes = 4;
-__host__ __device__ constexpr int tile_steps(int sh) { return sh == 0 ? 16 : sh == 1 ? 64 : 256; }
+//   4: 32 x 32 (256 threads) medium columns (32 columns of one colour are 128
+//      contiguous bytes per row, so rows at any offset waste less of each
+//      fetched line)
+constexpr int kNumTileShapes = 5;
+__host__ __device__ constexpr int tile_steps(int sh) {
+  return sh == 0 ? 16 : sh == 1 ? 64 : sh == 4 ? 32 : 256;
+}
 __host__ __device__ constexpr int tile_lines(int sh) {
-  return sh == 0 ? 64 : sh == 1 ? 16 : sh == 2 ? 4 : 16;
+  return sh == 0 ? 64 : sh == 1 ? 16 : sh == 2 ? 4 : sh == 3 ? 16 : 32;
 }
 __host__ __device__ constexpr int tile_threads(int sh) { return sh == 3 ? 1024 : 256; }
+// Rank of a shape by the length of the lines it takes (the longest lines are
+// the critical path of a sweep).
+__host__ __device__ constexpr int shape_rank(int sh) { return sh == 4 ? 1 : sh; }
 // Tiles of the eliminated right-hand side d that a line sweep keeps in LDS
 // (the last ones of each line; earlier tiles are parked in x and read back by
 // the backward pass), for values of tbytes bytes. Sized so that the LDS of a

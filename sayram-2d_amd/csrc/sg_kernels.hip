@@ -1899,7 +1899,8 @@ void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int n
     if (shape == 0) SG_FAC(T, DIR, 0);                     \
     else if (shape == 1) SG_FAC(T, DIR, 1);                \
     else if (shape == 2) SG_FAC(T, DIR, 2);                \
-    else SG_FAC(T, DIR, 3);                                \
+    else if (shape == 3) SG_FAC(T, DIR, 3);                \
+    else SG_FAC(T, DIR, 4);                                \
   } while (0)
   if (mixed) { if (dir == 0) SG_FAC_S(float, 0); else SG_FAC_S(float, 1); }
   else { if (dir == 0) SG_FAC_S(double, 0); else SG_FAC_S(double, 1); }
@@ -1930,7 +1931,8 @@ void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups,
   if (shape == 0) launch_line_s<T, 0>(s, d, m, ngroups, l, dir, zero, xid, bid);
   else if (shape == 1) launch_line_s<T, 1>(s, d, m, ngroups, l, dir, zero, xid, bid);
   else if (shape == 2) launch_line_s<T, 2>(s, d, m, ngroups, l, dir, zero, xid, bid);
-  else launch_line_s<T, 3>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else if (shape == 3) launch_line_s<T, 3>(s, d, m, ngroups, l, dir, zero, xid, bid);
+  else launch_line_s<T, 4>(s, d, m, ngroups, l, dir, zero, xid, bid);
 }
 template void launch_line<double>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
                                   int, bool, int, int);

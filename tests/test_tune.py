@@ -40,7 +40,8 @@ def test_tile_thresholds_change_rounding_only(gpu_lib):
     same trajectory up to rounding, and both stay on the oracle."""
     p = sc.seeded_case(60, 90, seed=71, scale=2.0)
     fs = []
-    # defaults (16x64 tiles here), 4x256 rows with 16x256 columns, 64x16 everywhere
+    # defaults (16x64 rows, 32x32 columns here), 4x256 rows with 16x256 columns,
+    # 64x16 everywhere
     for tiles in ((0, 0, 0, 0), (16, 40, 16, 64), (400, 400, 500, 500)):
         s = gpu_lib.Solver(p, gpu_lib.make_opts(precision=sg.SG_PC32, line_tiles=tiles))
         s.update(3)
