@@ -53,7 +53,7 @@ __host__ __device__ constexpr int shape_rank(int sh) { return sh == 4 ? 1 : sh; 
 // the backward pass), for values of tbytes bytes. Sized so that the LDS of a
 // workgroup does not lower the occupancy its registers allow.
 __host__ __device__ constexpr int ring_tiles(int sh, int tbytes) {
-  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 ? 6 : 4) : (sh == 3 ? 1 : 2);
+  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 || sh == 4 ? 6 : 4) : (sh == 3 ? 1 : 2);
 }
 
 __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
