@@ -643,10 +643,10 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
   };
   const int sk = tid / W, sc = tid % W;
   const bool sk_ok = sk < cnt;
-  if (tid < L) s_c[tid] = 0.0;
-  for (int t0 = 0; t0 < n; t0 += S) {
+  double vl[Q], vd[Q], vu[Q];
+  // every load of a tile is issued at once, into registers
+  auto load = [&](int t0) {
     const int tw = min(S, n - t0);
-    double vl[Q], vd[Q], vu[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = ek(q), u = eu(q);
@@ -659,12 +659,20 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
         vu[q] = t < n - 1 ? ld32(up, K) : 0.0;
       }
     }
+  };
+  if (tid < L) s_c[tid] = 0.0;
+  load(0);
+  for (int t0 = 0; t0 < n; t0 += S) {
+    const int tw = min(S, n - t0);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       s_lo[tslot<S>(ek(q), eu(q))] = vl[q];
       s_d[tslot<S>(ek(q), eu(q))] = vd[q];
       s_up[tslot<S>(ek(q), eu(q))] = vu[q];
     }
+    // The next tile's loads fly while this one is factored and stored: they
+    // read the bands, which this kernel does not write.
+    if (t0 + S < n) load(t0 + S);
     lds_barrier();
     // compose this thread's 4 maps: (P, Q) <- [[0, c], [-a, b]] (P, Q)
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
