@@ -148,13 +148,14 @@ struct Maps {
 // 1024 threads (16 columns per tile keep the accesses across the lines
 // coalesced with 4x fewer serial tiles). FP64 V-cycles keep the long columns
 // of the 9-point levels on 16 x 64 tiles, whose loads fit the registers of
-// 1024-thread tiles only in FP32. It depends only on the instance, its level
-// and the batch's precision, so results do not depend on the rest of the
-// batch.
+// 1024-thread tiles only in FP32, and their medium columns too (32 x 32 FP64
+// tiles need twice the registers and keep half the steps on chip). It depends
+// only on the instance, its level and the batch's precision, so results do
+// not depend on the rest of the batch.
 static int tile_shape(const int (&lim)[4], int n, int dir, int l, bool fp64_vcycle) {
   if (n <= lim[2 * dir]) return 0;
   if (dir == 0) return n <= lim[1] ? 1 : 2;
-  if (n <= lim[3]) return 4;
+  if (n <= lim[3]) return fp64_vcycle ? 1 : 4;
   return (fp64_vcycle && l > 0) ? 1 : 3;
 }
 
