@@ -78,7 +78,9 @@ static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   I.bcv[0] = (double *)take((ny + 1) * D); I.bcv[1] = (double *)take((ny + 1) * D);
   I.bcv[2] = (double *)take((nx + 1) * D); I.bcv[3] = (double *)take((nx + 1) * D);
   I.partial = (double *)take((size_t)4 * I.nblk0 * D);
-  for (int v = 0; v < kNumVec; ++v) I.kv[v] = (v == VXK && !mixed) ? nullptr : take(N * KT);
+  // PC32 keeps its shadow residual in FP32 (PV_RH) instead of VRH
+  for (int v = 0; v < kNumVec; ++v)
+    I.kv[v] = (v == VXK && !mixed) || (v == VRH && prec == SG_PC32) ? nullptr : take(N * KT);
   for (int v = 0; v < kNumVecAll - kNumVec; ++v)
     I.pv[v] = prec == SG_PC32 ? (float *)take(N * sizeof(float)) : nullptr;
   for (int l = 0; l < kMaxLev; ++l) {

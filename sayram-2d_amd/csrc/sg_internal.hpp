@@ -81,7 +81,7 @@ enum Vec { VR = 0, VRH, VP, VV, VS, VT, VPH, VSH, VXK, kNumVec };
 // FP32 level-0 vectors of an FP64 Krylov solve with an FP32 V-cycle: the
 // V-cycle input (p or s, scaled) and its outputs for p and s. Ids continue
 // the Krylov slots, so a V-cycle addresses either kind by one index.
-enum PcVec { PV_B = kNumVec, PV_PH, PV_SH, kNumVecAll };
+enum PcVec { PV_B = kNumVec, PV_PH, PV_SH, PV_RH, kNumVecAll };
 
 // solver status per instance
 enum Status : int { ST_ACTIVE = 0, ST_HALF = 1, ST_DONE = 2, ST_FAIL = 3, ST_IDLE = 4 };
@@ -132,7 +132,7 @@ struct InstDev {
   double *f, *R, *x, *r64;
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
-  float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle input and outputs
+  float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle input and outputs, shadow residual
   double *partial;            // 4 * nblk0
   unsigned int *ticket;       // 2 tickets
   KScal *sc;

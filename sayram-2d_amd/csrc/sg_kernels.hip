@@ -396,7 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
       const double r = start_residual(R, d, fK, bw, f3[1][0], be, f3[1][2], bs, f3[0][1], bn,
                                       f3[2][1]);
       gbl(ptr<double>(I.kv[VR]))[K] = r;
-      gbl(ptr<double>(I.kv[VRH]))[K] = r;
+      if (I.prec != SG_PC32) gbl(ptr<double>(I.kv[VRH]))[K] = r;   // PC32: the first k_bicg_p
       gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
       gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
       part[0] = r * r;
@@ -1463,7 +1463,11 @@ __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, siz
   return gbl(cptr<T>(I.kv[id]))[K];
 }
 
-// p = r + beta (p - omega v); PC: also the scaled FP32 V-cycle input
+// p = r + beta (p - omega v); PC: also the scaled FP32 V-cycle input, which
+// in the first iteration (beta = 0, p = r0) is also stored as the shadow
+// residual rh = fl32(r0 / ||r0||). BiCGSTAB admits any fixed rh with
+// (rh, r0) != 0; this one is r0's direction, read in FP32 by every later
+// iteration, and rho0 = (rh, r0) = ||r0|| to FP32 rounding.
 template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
@@ -1474,10 +1478,22 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
   if (K >= I.nx * I.ny) return;
   const T beta = (T)rsc(I)->beta, omega = (T)rsc(I)->omega;
   const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
+  const bool first = PC && rsc(I)->iters == 0;
   T *p = ptr<T>(I.kv[VP]);
   const T pv = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
   gbl(p)[K] = pv;
-  if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(pv / pcs);
+  if (PC) {
+    const float p32 = (float)(pv / pcs);
+    gbl(I.pv[PV_B - kNumVec])[K] = p32;
+    if (first) gbl(I.pv[PV_RH - kNumVec])[K] = p32;
+  }
+}
+
+// the shadow residual rh at slot K (PC: FP32, see k_bicg_p)
+template <typename T, bool PC>
+__device__ __forceinline__ T shadow(const InstDev &I, size_t K) {
+  if (PC) return (T)gbl(I.pv[PV_RH - kNumVec])[K];
+  return gbl(cptr<T>(I.kv[VRH]))[K];
 }
 
 // v = M ph; partial (rh, v)
@@ -1493,7 +1509,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
   if (K < N) {
     int i, j;
     cell_ij(K, I.nx, half_x(I.nx), i, j);
-    const T rh = gbl(cptr<T>(I.kv[VRH]))[K];   // loaded before the store below
+    const T rh = shadow<T, PC>(I, K);   // loaded before the store below
     const T v = PC ? (T)(rsc(I)->pcs * spmv64(I, K, i, j, cptr<float>(I.pv[PV_PH - kNumVec])))
                    : spmv5<T>(I, K, i, j, cptr<T>(I.kv[VPH]));
     gbl(ptr<T>(I.kv[VV]))[K] = v;
@@ -1570,7 +1586,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_x(const InstDev *__restrict__ i
     const T ph = precond_vec<T, PC>(I, VPH, PV_PH, K);
     const T sh = precond_vec<T, PC>(I, VSH, PV_SH, K);
     const T sv = gbl(cptr<T>(I.kv[VS]))[K], tv = gbl(cptr<T>(I.kv[VT]))[K];
-    const T rh = gbl(cptr<T>(I.kv[VRH]))[K];
+    const T rh = shadow<T, PC>(I, K);
     const bool act = st != ST_HALF;
     const T xh = xv + alpha * ph;
     const T r = sv - omega * tv;
@@ -1727,10 +1743,12 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
     double t[2];
     sum_partials<double, 2>(I, t);
     if (threadIdx.x == 0) {
-      s.rho = t[0]; s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
+      s.pcs = t[0] > 0.0 ? sqrt(t[0]) : 1.0;
+      // PC32: the shadow residual is r0 / ||r0|| (k_bicg_p), so rho0 = ||r0||
+      s.rho = I.prec == SG_PC32 ? t[0] / s.pcs : t[0];
+      s.rho_old = 1.0; s.alpha = 1.0; s.omega = 1.0; s.beta = 0.0;
       s.rnorm2 = t[0]; s.bnorm2 = t[1]; s.iters = 0;
       s.best2 = t[0]; s.best_it = 0;
-      s.pcs = t[0] > 0.0 ? sqrt(t[0]) : 1.0;
       if (t[0] <= s.tol2 * t[1]) { s.status = ST_DONE; atomicSub(cnt, 1); }
     }
   } else if (MODE == F_V) {   // after the V-cycle of p (active instances only)
