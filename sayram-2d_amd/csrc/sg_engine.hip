@@ -78,9 +78,11 @@ static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   I.bcv[0] = (double *)take((ny + 1) * D); I.bcv[1] = (double *)take((ny + 1) * D);
   I.bcv[2] = (double *)take((nx + 1) * D); I.bcv[3] = (double *)take((nx + 1) * D);
   I.partial = (double *)take((size_t)4 * I.nblk0 * D);
-  // PC32 keeps its shadow residual in FP32 (PV_RH) instead of VRH
+  // PC32 keeps its shadow residual and its search direction in FP32 (PV_RH,
+  // PV_P) instead of VRH and VP
   for (int v = 0; v < kNumVec; ++v)
-    I.kv[v] = (v == VXK && !mixed) || (v == VRH && prec == SG_PC32) ? nullptr : take(N * KT);
+    I.kv[v] = (v == VXK && !mixed) || ((v == VRH || v == VP) && prec == SG_PC32) ? nullptr
+                                                                                 : take(N * KT);
   for (int v = 0; v < kNumVecAll - kNumVec; ++v)
     I.pv[v] = prec == SG_PC32 ? (float *)take(N * sizeof(float)) : nullptr;
   for (int l = 0; l < kMaxLev; ++l) {
@@ -811,7 +813,7 @@ template <typename KT, typename VT> void Batch::iteration(bool xstate) {
   const BlkMap *m0 = dm.cmap[0];
   constexpr bool pc = sizeof(KT) != sizeof(VT);
   tbegin(8); launch_bicg<KT>(stream, 0, d_insts, m0, nb, xstate, pc); tend(8);
-  vcycle<VT>(0, pc ? PV_PH : VPH, pc ? PV_B : VP);
+  vcycle<VT>(0, pc ? PV_PH : VPH, pc ? PV_P : VP);
   tbegin(6); launch_bicg<KT>(stream, 1, d_insts, m0, nb, xstate, pc); tend(6);
   launch_finish<KT>(stream, FIN_V, d_insts, n, d_cnt);
   tbegin(8); launch_bicg<KT>(stream, 2, d_insts, m0, nb, xstate, pc); tend(8);
@@ -1121,6 +1123,16 @@ struct sg_batch {
     };
     const bool busy[2] = {inited[0] && has(0), inited[1] && has(1)};
     if (!(busy[0] && busy[1])) return busy[1] ? run(1) : busy[0] ? run(0) : SG_OK;
+    // Rebuilding a group's maps frees and allocates device memory, and a free
+    // synchronizes the device, which would invalidate a graph the other
+    // group's thread is capturing. So both groups are rebuilt here, before
+    // they step concurrently; the concurrent steps then only launch, capture
+    // and wait on their own streams.
+    for (int k = 0; k < 2; ++k)
+      if (g[k].dirty) {
+        const int rc = g[k].rebuild();
+        if (rc) return rc;
+      }
     int rc1 = SG_OK;
     std::string err1;
     std::thread worker([&] {

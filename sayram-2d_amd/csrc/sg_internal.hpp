@@ -81,7 +81,7 @@ enum Vec { VR = 0, VRH, VP, VV, VS, VT, VPH, VSH, VXK, kNumVec };
 // FP32 level-0 vectors of an FP64 Krylov solve with an FP32 V-cycle: the
 // V-cycle input (p or s, scaled) and its outputs for p and s. Ids continue
 // the Krylov slots, so a V-cycle addresses either kind by one index.
-enum PcVec { PV_B = kNumVec, PV_PH, PV_SH, PV_RH, kNumVecAll };
+enum PcVec { PV_B = kNumVec, PV_P, PV_PH, PV_SH, PV_RH, kNumVecAll };
 
 // solver status per instance
 enum Status : int { ST_ACTIVE = 0, ST_HALF = 1, ST_DONE = 2, ST_FAIL = 3, ST_IDLE = 4 };
@@ -92,6 +92,8 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   double scale;          // mixed: norm of the FP64 residual being corrected
   double pcs;            // FP32 V-cycle of an FP64 solve: scale of its input
                          // and output, the current residual norm
+  double pcs_p;          // PC32: the scale the stored FP32 direction p was
+                         // written with (pcs of the previous iteration)
   double onorm2, obnorm2, otol2;   // mixed: outer (FP64) residual norms
   double oprev2;         // mixed: residual norm^2 of the previous refinement
   double best2;          // smallest Krylov residual norm^2 of this solve
@@ -132,7 +134,8 @@ struct InstDev {
   double *f, *R, *x, *r64;
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
-  float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle input and outputs, shadow residual
+  float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle inputs (s, p) and outputs,
+                                     // shadow residual
   double *partial;            // 4 * nblk0
   unsigned int *ticket;       // 2 tickets
   KScal *sc;

@@ -396,9 +396,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
       const double r = start_residual(R, d, fK, bw, f3[1][0], be, f3[1][2], bs, f3[0][1], bn,
                                       f3[2][1]);
       gbl(ptr<double>(I.kv[VR]))[K] = r;
-      if (I.prec != SG_PC32) gbl(ptr<double>(I.kv[VRH]))[K] = r;   // PC32: the first k_bicg_p
-      gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
-      gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
+      if (I.prec != SG_PC32) {   // PC32: the first k_bicg_p sets rh and p, from r alone
+        gbl(ptr<double>(I.kv[VRH]))[K] = r;
+        gbl(ptr<double>(I.kv[VP]))[K] = 0.0;
+        gbl(ptr<double>(I.kv[VV]))[K] = 0.0;
+      }
       part[0] = r * r;
       part[1] = R * R;
     }
@@ -1463,11 +1465,16 @@ __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, siz
   return gbl(cptr<T>(I.kv[id]))[K];
 }
 
-// p = r + beta (p - omega v); PC: also the scaled FP32 V-cycle input, which
-// in the first iteration (beta = 0, p = r0) is also stored as the shadow
-// residual rh = fl32(r0 / ||r0||). BiCGSTAB admits any fixed rh with
-// (rh, r0) != 0; this one is r0's direction, read in FP32 by every later
-// iteration, and rho0 = (rh, r0) = ||r0|| to FP32 rounding.
+// p = r + beta (p - omega v).
+// PC: p is stored only in FP32, divided by the residual norm pcs (it is the
+// V-cycle input P = p / pcs). p is a search direction: x moves along
+// ph = M^-1 P and r along v = A ph, both from the same ph, so the rounding of
+// P changes the direction (like the FP32 V-cycle does), never the agreement
+// of r with R - A x. The first iteration (p = r0) ignores the stored P and v,
+// and also stores P as the shadow residual rh = fl32(r0 / ||r0||): BiCGSTAB
+// admits any fixed rh with (rh, r0) != 0, so rh is r0's direction, read in
+// FP32 by every later iteration, with rho0 = (rh, r0) = ||r0|| to FP32
+// rounding.
 template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
@@ -1477,15 +1484,19 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ i
   const int K = m.k0 + threadIdx.x;
   if (K >= I.nx * I.ny) return;
   const T beta = (T)rsc(I)->beta, omega = (T)rsc(I)->omega;
-  const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
-  const bool first = PC && rsc(I)->iters == 0;
-  T *p = ptr<T>(I.kv[VP]);
-  const T pv = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
-  gbl(p)[K] = pv;
   if (PC) {
+    const double pcs = rsc(I)->pcs, pcs_p = rsc(I)->pcs_p;   // read before the stores
+    const bool first = rsc(I)->iters == 0;
+    float *P = I.pv[PV_P - kNumVec];
+    const T r = gbl(cptr<T>(I.kv[VR]))[K], v = gbl(cptr<T>(I.kv[VV]))[K];
+    const T pold = (T)(pcs_p * (double)gbl(P)[K]);
+    const T pv = first ? r : r + beta * (pold - omega * v);
     const float p32 = (float)(pv / pcs);
-    gbl(I.pv[PV_B - kNumVec])[K] = p32;
+    gbl(P)[K] = p32;
     if (first) gbl(I.pv[PV_RH - kNumVec])[K] = p32;
+  } else {
+    T *p = ptr<T>(I.kv[VP]);
+    gbl(p)[K] = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
   }
 }
 
@@ -1779,6 +1790,7 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
         s.rho_old = s.rho;
         s.rho = (double)t[0];
         s.rnorm2 = (double)t[1];
+        s.pcs_p = s.pcs;
         s.pcs = s.rnorm2 > 0.0 ? sqrt(s.rnorm2) : 1.0;
         s.beta = (double)(((T)s.rho / (T)s.rho_old) * ((T)s.alpha / (T)s.omega));
         // Converged; or stalled at the rounding floor: no halving of the
