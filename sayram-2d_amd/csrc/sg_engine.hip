@@ -164,19 +164,19 @@ static int tile_shape(const int (&lim)[4], int n, int dir, int l, bool fp64_vcyc
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
-// b % 8), each with its own L2. Two neighbouring column groups of one colour
-// share every cache line of their rows (16 columns of 4 bytes fill half a
-// 128-byte line), so in each block of 16 dispatch slots the pair (2x, 2x+1) of
-// the map goes to slots x and x + 8: the same XCD, dispatched together, one
-// fetch per line. The order of the pairs is kept.
-template <typename G> static void xcd_pairs(std::vector<G> &v) {
-  constexpr int kXcd = 8;
+// b % 8), each with its own L2. Neighbouring column groups of one colour share
+// the cache lines at their common edge (a group's run of each row starts at an
+// arbitrary offset), so in each block of 8 C dispatch slots the chunk of C
+// consecutive map entries C x .. C x + C - 1 goes to slots x, x + 8, ..: the
+// same XCD, dispatched together, one fetch per shared line. The order of the
+// chunks is kept. C = 4 (measured: 0.4% faster than pairs on the batched
+// workload, equal to C = 8).
+template <typename G> static void xcd_chunks(std::vector<G> &v) {
+  constexpr int kXcd = 8, C = 4;
   std::vector<G> out(v);
-  for (size_t s = 0; s + 2 * kXcd <= v.size(); s += 2 * kXcd)
-    for (int x = 0; x < kXcd; ++x) {
-      out[s + x] = v[s + 2 * x];
-      out[s + kXcd + x] = v[s + 2 * x + 1];
-    }
+  for (size_t s = 0; s + C * kXcd <= v.size(); s += C * kXcd)
+    for (int x = 0; x < kXcd; ++x)
+      for (int c = 0; c < C; ++c) out[s + c * kXcd + x] = v[s + C * x + c];
   v.swap(out);
 }
 
@@ -612,7 +612,7 @@ int Batch::rebuild() {
           for (auto *w : {&v, &M.sw[l][d][p][sh]}) {
             std::stable_sort(w->begin(), w->end(),
                              [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
-            if (d == 1) xcd_pairs(*w);
+            if (d == 1) xcd_chunks(*w);
           }
         }
   // fused cycles with the most levels first: they are the critical path
