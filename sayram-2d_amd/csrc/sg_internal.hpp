@@ -50,10 +50,14 @@ __host__ __device__ constexpr int tile_threads(int sh) { return sh == 3 ? 1024 :
 __host__ __device__ constexpr int shape_rank(int sh) { return sh == 4 ? 1 : sh; }
 // Tiles of the eliminated right-hand side d that a line sweep keeps in LDS
 // (the last ones of each line; earlier tiles are parked in x and read back by
-// the backward pass), for values of tbytes bytes. Sized so that the LDS of a
-// workgroup does not lower the occupancy its registers allow.
-__host__ __device__ constexpr int ring_tiles(int sh, int tbytes) {
-  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 || sh == 4 ? 6 : 4) : (sh == 3 ? 1 : 2);
+// the backward pass), for values of tbytes bytes and an nb-point operator.
+// Sized so that the LDS of a workgroup does not lower the occupancy its
+// registers allow, except for the 5-point 32 x 32 tiles: 9 tiles (288 steps)
+// at 3 workgroups per CU measured faster than 6 at 4 (fewer parked tiles of the
+// 512-cell columns).
+__host__ __device__ constexpr int ring_tiles(int sh, int tbytes, int nb) {
+  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 ? 6 : sh == 4 ? (nb == 9 ? 8 : 9) : 4)
+                     : (sh == 3 ? 1 : 2);
 }
 
 __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }

@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
   constexpr int S = tile_steps(SH), L = tile_lines(SH), NT = tile_threads(SH);
   constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
   constexpr int Q = S * L / NT;      // elements per thread in the loads
-  constexpr int KT = ring_tiles(SH, sizeof(T));   // tiles of d kept in LDS
+  constexpr int KT = ring_tiles(SH, sizeof(T), NB);   // tiles of d kept in LDS
   constexpr int TB = L * (S + 4);                  // one tile buffer
   static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
   __shared__ T s_rb[KT * TB];   // r, then d, then x of a tile, in a ring of KT
