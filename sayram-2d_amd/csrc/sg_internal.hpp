@@ -54,10 +54,17 @@ __host__ __device__ constexpr int shape_rank(int sh) { return sh == 4 ? 1 : sh; 
 // Sized so that the LDS of a workgroup does not lower the occupancy its
 // registers allow, except for the 5-point 32 x 32 tiles: 9 tiles (288 steps)
 // at 3 workgroups per CU measured faster than 6 at 4 (fewer parked tiles of the
-// 512-cell columns).
+// 512-cell columns). The 5-point long-line shapes (2, 3) hold one tile more
+// than the 9-point ones, whose registers allow fewer workgroups.
 __host__ __device__ constexpr int ring_tiles(int sh, int tbytes, int nb) {
-  return tbytes == 4 ? (sh == 0 ? 2 : sh == 3 ? 6 : sh == 4 ? (nb == 9 ? 8 : 9) : 4)
-                     : (sh == 3 ? 1 : 2);
+  if (tbytes != 4) return sh == 3 ? 1 : 2;
+  switch (sh) {
+    case 0: return 2;
+    case 2: return nb == 9 ? 4 : 5;
+    case 3: return nb == 9 ? 6 : 7;
+    case 4: return nb == 9 ? 8 : 9;
+    default: return 4;
+  }
 }
 
 __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
