@@ -1095,7 +1095,8 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
       // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
       const int slot = (da < 0 ? 1 : 0) + 2 * (db < 0 ? 1 : 0);
       const auto w = slot == 0 ? pw0 : slot == 1 ? pw1 : slot == 2 ? pw2 : pw3;
-      wv[db + 1][da + 1] = w[kf];
+      // the fine point at C itself is injected: k_pweights stores exactly 1
+      wv[db + 1][da + 1] = (da == 0 && db == 0) ? T(1) : T(w[kf]);
       rv[db + 1][da + 1] = r[kf];
       ok[db + 1][da + 1] = okj && oki;
     }
