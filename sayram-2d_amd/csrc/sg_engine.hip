@@ -212,6 +212,7 @@ struct Batch {
   // the others on `stream`; `stream` waits for it before the next sweep.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_lev[kMaxLev] = {};   // with `side`: level l's operator is built
   hipError_t fan_err = hipSuccess;   // first failure of the fork/join calls
   std::vector<Slot> slots;
   int capacity = 0;
@@ -288,6 +289,8 @@ Batch::~Batch() {
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  for (auto e : ev_lev)
+    if (e) (void)hipEventDestroy(e);
   if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -324,6 +327,7 @@ int Batch::init(const sg_opts *o, size_t b) {
     HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    for (auto &e : ev_lev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   HIPCHK(hipMalloc(&d_cnt, 2 * sizeof(int)));
   HIPCHK(hipHostMalloc((void **)&h_cnt, 2 * sizeof(int), hipHostMallocDefault));
@@ -898,24 +902,46 @@ int Batch::step_once() {
                mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
   const int nb0 = dm.ncmap[0];
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
-  for (int l = 0; l < dm.maxlev; ++l) {
-    tbegin(1);
-    // LU factors of the line matrices: smoothing lines of both colours and the
-    // exact solve on the coarsest level
+  // LU factors of the line matrices of level l: smoothing lines of both
+  // colours and the exact solve on the coarsest level
+  auto factors = [&](int l, hipStream_t s) {
     for (int d = 0; d < 2; ++d)
       for (int p = 0; p < 2; ++p)
-        fan_out(dm.ngmap[l][d][p], [&](hipStream_t s, int sh) {
+        for (int sh = 0; sh < kNumTileShapes; ++sh)
           launch_line_factor(s, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d, sh, h32);
-        });
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      launch_line_factor(stream, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
-      launch_line_factor(stream, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
+      launch_line_factor(s, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
+      launch_line_factor(s, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
     }
-    if (l + 1 < dm.maxlev) {
+  };
+  if (side && time_id < 0) {
+    // A level's factorisations depend only on its operator, and nothing else
+    // in the set-up reads them: they run on the side stream, each level's as
+    // soon as its operator exists, beside the chain of prolongation weights and
+    // Galerkin products that builds the next level on the main stream. The
+    // main stream waits for them before the solve. Same kernels, same inputs.
+    HIPCHK(hipEventRecord(ev_lev[0], stream));
+    for (int l = 0; l + 1 < dm.maxlev; ++l) {
       launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
       launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
+      HIPCHK(hipEventRecord(ev_lev[l + 1], stream));
     }
-    tend(1);
+    for (int l = 0; l < dm.maxlev; ++l) {
+      HIPCHK(hipStreamWaitEvent(side, ev_lev[l], 0));
+      factors(l, side);
+    }
+    HIPCHK(hipEventRecord(ev_join, side));
+    HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+  } else {
+    for (int l = 0; l < dm.maxlev; ++l) {
+      tbegin(1);
+      factors(l, stream);
+      if (l + 1 < dm.maxlev) {
+        launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
+        launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
+      }
+      tend(1);
+    }
   }
   if (!mixed) {
     // the assembly started BiCGSTAB (r = rh = R - M f, p = v = 0, partials)
