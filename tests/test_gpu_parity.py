@@ -142,17 +142,21 @@ def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
             assert rel_l2(out[fc][k], ref) <= TOL
 
 
+@pytest.mark.parametrize("precision", ["pc32", "fp64"])
 @pytest.mark.parametrize("graph", [1, -1], ids=["graph", "eager"])
-def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph):
-    """The tile shapes of a sweep (and of the per-step factorisations) run on
-    two streams by default; they solve disjoint lines, so every result equals
-    the one-stream schedule bit for bit. Line lengths 12, 100, 300 and 900 give
-    sweeps with three tile shapes in both directions."""
+def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph, precision):
+    """The tile shapes of a sweep run on two streams by default, and the
+    per-step line factorisations run on the second stream beside the chain of
+    Galerkin products; they write disjoint data, so every result equals the
+    one-stream schedule bit for bit. Line lengths 12, 100, 300 and 900 give
+    sweeps with three tile shapes in both directions (FP64 V-cycles use other
+    shapes than FP32 ones)."""
+    prec = {"pc32": gpu_lib.SG_PC32, "fp64": gpu_lib.SG_FP64}[precision]
     probs = [sc.seeded_case(nx, ny, seed=120 + k) for k, (nx, ny) in
              enumerate([(12, 300), (100, 12), (300, 100), (900, 40), (60, 900)])]
     out = {}
     for ns in (1, 2):
-        b = gpu_lib.Batch(gpu_lib.make_opts(precision=gpu_lib.SG_PC32, streams=ns, use_graph=graph))
+        b = gpu_lib.Batch(gpu_lib.make_opts(precision=prec, streams=ns, use_graph=graph))
         slots = [b.add(p) for p in probs]
         b.step(3)
         out[ns] = [b.f(s) for s in slots]
