@@ -26,6 +26,7 @@ constexpr int kMaxLev = 16;
 // may address it with 32-bit byte offsets from its base.
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
+constexpr int kLightBlocks = 2; // map entries per workgroup of the lightest vector kernels
 // Tile shapes of the line solves: L lines x S steps staged through LDS by a
 // workgroup of NT threads, each thread owning 4 consecutive steps of a line in
 // the solve:

@@ -1476,28 +1476,39 @@ __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, siz
 // admits any fixed rh with (rh, r0) != 0, so rh is r0's direction, read in
 // FP32 by every later iteration, with rho0 = (rh, r0) = ||r0|| to FP32
 // rounding.
+//
+// A workgroup takes two map entries (kLightBlocks): the chain of dependent
+// reads in front of an entry's vector traffic (map entry, instance, its
+// scalars) is latency this light kernel cannot hide within one 256-cell
+// block, and the second entry's chain is issued while the first one's vectors
+// move (the map, the instance table and the scalars are read-only here).
 template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ insts,
-                                                  const BlkMap *__restrict__ map) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (rsc(I)->status != ST_ACTIVE) return;
-  const int K = m.k0 + threadIdx.x;
-  if (K >= I.nx * I.ny) return;
-  const T beta = (T)rsc(I)->beta, omega = (T)rsc(I)->omega;
-  if (PC) {
-    const double pcs = rsc(I)->pcs, pcs_p = rsc(I)->pcs_p;   // read before the stores
-    const bool first = rsc(I)->iters == 0;
-    float *P = I.pv[PV_P - kNumVec];
-    const T r = gbl(cptr<T>(I.kv[VR]))[K], v = gbl(cptr<T>(I.kv[VV]))[K];
-    const T pold = (T)(pcs_p * (double)gbl(P)[K]);
-    const T pv = first ? r : r + beta * (pold - omega * v);
-    const float p32 = (float)(pv / pcs);
-    gbl(P)[K] = p32;
-    if (first) gbl(I.pv[PV_RH - kNumVec])[K] = p32;
-  } else {
-    T *p = ptr<T>(I.kv[VP]);
-    gbl(p)[K] = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
+                                                  const BlkMap *__restrict__ map, int nmap) {
+#pragma unroll
+  for (int e = 0; e < kLightBlocks; ++e) {
+    const int bi = kLightBlocks * blockIdx.x + e;
+    if (bi >= nmap) break;
+    const BlkMap m = map[bi];
+    const InstDev &I = insts[m.inst];
+    if (rsc(I)->status != ST_ACTIVE) continue;
+    const int K = m.k0 + threadIdx.x;
+    if (K >= I.nx * I.ny) continue;
+    const T beta = (T)rsc(I)->beta, omega = (T)rsc(I)->omega;
+    if (PC) {
+      const double pcs = rsc(I)->pcs, pcs_p = rsc(I)->pcs_p;   // read before the stores
+      const bool first = rsc(I)->iters == 0;
+      float *P = I.pv[PV_P - kNumVec];
+      const T r = gbl(cptr<T>(I.kv[VR]))[K], v = gbl(cptr<T>(I.kv[VV]))[K];
+      const T pold = (T)(pcs_p * (double)gbl(P)[K]);
+      const T pv = first ? r : r + beta * (pold - omega * v);
+      const float p32 = (float)(pv / pcs);
+      gbl(P)[K] = p32;
+      if (first) gbl(I.pv[PV_RH - kNumVec])[K] = p32;
+    } else {
+      T *p = ptr<T>(I.kv[VP]);
+      gbl(p)[K] = gbl(cptr<T>(I.kv[VR]))[K] + beta * (gbl(p)[K] - omega * gbl(cptr<T>(I.kv[VV]))[K]);
+    }
   }
 }
 
@@ -1533,21 +1544,26 @@ __global__ void __launch_bounds__(kBlock) k_bicg_v(const InstDev *__restrict__ i
 // s = r - alpha v; partial (s, s); PC: also the scaled FP32 V-cycle input
 template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ insts,
-                                                  const BlkMap *__restrict__ map) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (rsc(I)->status != ST_ACTIVE) return;
-  const int K = m.k0 + threadIdx.x;
-  const int N = I.nx * I.ny;
-  T acc[1] = {T(0)};
-  if (K < N) {
-    const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
-    const T s = gbl(cptr<T>(I.kv[VR]))[K] - (T)rsc(I)->alpha * gbl(cptr<T>(I.kv[VV]))[K];
-    gbl(ptr<T>(I.kv[VS]))[K] = s;
-    if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(s / pcs);
-    acc[0] = s * s;
+                                                  const BlkMap *__restrict__ map, int nmap) {
+#pragma unroll
+  for (int e = 0; e < kLightBlocks; ++e) {   // two map entries, as k_bicg_p
+    const int bi = kLightBlocks * blockIdx.x + e;
+    if (bi >= nmap) break;
+    const BlkMap m = map[bi];
+    const InstDev &I = insts[m.inst];
+    if (rsc(I)->status != ST_ACTIVE) continue;   // uniform over the workgroup
+    const int K = m.k0 + threadIdx.x;
+    const int N = I.nx * I.ny;
+    T acc[1] = {T(0)};
+    if (K < N) {
+      const double pcs = PC ? rsc(I)->pcs : 1.0;   // read before the stores
+      const T s = gbl(cptr<T>(I.kv[VR]))[K] - (T)rsc(I)->alpha * gbl(cptr<T>(I.kv[VV]))[K];
+      gbl(ptr<T>(I.kv[VS]))[K] = s;
+      if (PC) gbl(I.pv[PV_B - kNumVec])[K] = (float)(s / pcs);
+      acc[0] = s * s;
+    }
+    store_partials<T, 1>(I, m.k0 / kBlock, acc);
   }
-  store_partials<T, 1>(I, m.k0 / kBlock, acc);
 }
 
 // t = M sh; partials (t, s), (t, t)
@@ -2020,9 +2036,9 @@ template <typename T, bool PC>
 static void launch_bicg_t(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb,
                           bool xstate) {
   switch (which) {
-    case 0: k_bicg_p<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 0: k_bicg_p<T, PC><<<(nb + kLightBlocks - 1) / kLightBlocks, kBlock, 0, s>>>(d, m, nb); break;
     case 1: k_bicg_v<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
-    case 2: k_bicg_s<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
+    case 2: k_bicg_s<T, PC><<<(nb + kLightBlocks - 1) / kLightBlocks, kBlock, 0, s>>>(d, m, nb); break;
     case 3: k_bicg_t<T, PC><<<nb, kBlock, 0, s>>>(d, m); break;
     default:
       if (xstate) k_bicg_x<T, true, PC><<<nb, kBlock, 0, s>>>(d, m);
