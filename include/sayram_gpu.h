@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 2
+#define SG_ABI_VERSION 3
 
 typedef enum {
   SG_OK = 0,
@@ -111,19 +111,40 @@ typedef struct {
                            solves are latency-bound, the others' work
                            bandwidth-bound). 0 selects 2^21; -1 keeps one group.
                            Results are identical. */
-  int reserved[1];
+  int no_clamp;         /* 0 (default): the step ends with f = max(x, 0), so
+                           min f >= 0 holds exactly; the reference copies the
+                           solution unclamped (source/Solver.cc:279-284), whose
+                           negative entries can only be rounding-sized while M
+                           is an M-matrix. Cells clamped are counted
+                           (sg_step_stats.last_clamps, clamp_count): a count
+                           above 0 marks a step that departs from the
+                           reference. 1: f = x, as the reference. */
 } sg_opts;
 
+/* Acceptance of a step. The solve stops when ||R - M x||_2 <= rtol ||R||_2 (the
+ * BiCGSTAB recurrence residual, or the FP64 residual of each refinement in
+ * MIXED), or when that residual stops halving for 4 iterations (1 refinement)
+ * while below 1e-9 relative: the FP64 rounding floor, about cond(M) eps, which
+ * a direct solve reaches too. The step is then accepted only if the true
+ * residual ||R - M x|| of the solution is within max(rtol, 1e-9) ||R||, every
+ * x is finite and the solve did not break down or hit max_iters. A rejected
+ * step leaves f and the step count unchanged and returns SG_ERR_NOCONV (the
+ * reference's direct solve never produces an unconverged step). */
 typedef struct {
   int64_t steps;              /* steps taken since creation / last set_f */
   int last_iters;             /* Krylov iterations of the last step (sum of
                                  inner iterations for MIXED) */
   int last_refinements;       /* MIXED: FP64 refinement sweeps of the last step */
-  double last_rel_residual;   /* ||R - M f||/||R|| before clamping, last step */
-  double min_f;               /* min f over the grid after the last step (>= 0) */
+  double last_rel_residual;   /* ||R - M x||/||R|| of the last solution x */
+  double min_f;               /* min f over the grid after the last step (>= 0
+                                 unless no_clamp) */
   int64_t clamp_count;        /* cumulative count of cells clamped to 0 */
   int64_t total_iters;        /* cumulative Krylov iterations */
   int status;                 /* sg_status of the last step */
+  int last_nonfinite;         /* cells of the last solution that were not
+                                 finite (the step was then rejected) */
+  int64_t last_clamps;        /* cells clamped to 0 by the last step */
+  double last_min_x;          /* min of the last solution before clamping */
 } sg_step_stats;
 
 typedef struct sg_solver sg_solver;   /* one instance (Solver equivalent) */
@@ -144,7 +165,10 @@ int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, con
 /* Dirichlet vertex values of one side at the next step's time t
  * (fill_vertex_from_bcs, source/Solver.cc:385-422). */
 int sg_set_bc_values(sg_solver *h, int side, const double *values);
-/* Re-upload G and D after Equation::update (source/Solver.cc:287-288). */
+/* New G, D and/or 1/tau after Equation::update (source/Solver.cc:287-288,
+ * update_Lambda :57-65), (nx,ny) layout. A NULL array keeps its current values,
+ * so a refresh moves only what changed; Lambda = G*D is formed on the device.
+ * Non-finite values are rejected (SG_ERR_ARG) before anything is replaced. */
 int sg_set_coeffs(sg_solver *h, const sg_coeffs *coeffs);
 /* nsteps x Solver::update (source/Solver.cc:270-290); synchronous. */
 int sg_step(sg_solver *h, int nsteps);
@@ -175,6 +199,8 @@ int sg_batch_step_until(sg_batch *b, const int64_t *target_steps, int nslots, in
 int sg_batch_get_f(sg_batch *b, int slot, double *out);
 int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep);
 int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values);
+/* sg_set_coeffs for one instance of a batch. */
+int sg_batch_set_coeffs(sg_batch *b, int slot, const sg_coeffs *coeffs);
 int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out);
 int sg_batch_size(const sg_batch *b);
 size_t sg_batch_bytes_in_use(const sg_batch *b);
@@ -184,11 +210,19 @@ void sg_batch_destroy(sg_batch *b);
 /* Assembly only: writes the 5 FP64 bands (d|w|e|s|n, K = j*nx+i order) and R
  * of the next step into host buffers, for parity checks of the assembly. */
 int sg_debug_assemble(sg_solver *h, double *bands5, double *R);
-/* Kernel timing with HIP events on the library's stream: kernel_id selects the
- * kernel to bracket (see sg_kernel_name), -1 disables. */
+/* Kernel timing with HIP events around every dispatch of a kernel family, on
+ * the stream it runs on (the steps then launch eagerly): kernel_id selects the
+ * family (see sg_kernel_name), sg_kernel_count() selects every family, -1
+ * disables. Selecting resets the totals. */
 int sg_batch_time_kernel(sg_batch *b, int kernel_id);
+/* Totals over the selected families: time, dispatches and algorithmic bytes
+ * (each array a kernel reads or writes counted once per launch). */
 int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes);
+/* Totals of one family (bytes 0 where no byte model applies). */
+int sg_batch_kernel_time_id(sg_batch *b, int kernel_id, double *total_ms, int64_t *launches,
+                            double *bytes);
 const char *sg_kernel_name(int kernel_id);
+int sg_kernel_count(void);
 int sg_solver_batch(sg_solver *h, sg_batch **out);
 
 #ifdef __cplusplus

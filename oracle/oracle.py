@@ -135,3 +135,17 @@ def run(op: OracleProblem, f, nsteps):
     if rc:
         raise RuntimeError(f"orc_run failed: {rc}")
     return out
+
+
+def run_schedule(problems, f):
+    """Steps of a time-dependent Equation: step n uses problems[n], the inputs
+    the Equation holds after update(t_n) (G, D, 1/tau and the Dirichlet vertex
+    values at t_n; source/Solver.cc:286-289 refreshes Lambda and the vertex
+    table after each solve, for the next step). Returns the state after each
+    step."""
+    out = []
+    f = np.array(f, dtype=np.float64, copy=True)
+    for p in problems:
+        f = step(OracleProblem(p), f)
+        out.append(f)
+    return out

@@ -67,8 +67,8 @@ static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   I.nx = nx; I.ny = ny; I.nlev = (int)dims.size();
   I.nblk0 = (int)((N + kBlock - 1) / kBlock);
   I.prec = prec;
-  I.G = (double *)take(N * D); I.Lxx = (double *)take(N * D); I.Lxy = (double *)take(N * D);
-  I.Lyy = (double *)take(N * D); I.itau = (double *)take(N * D);
+  I.G = (double *)take(N * D); I.Dxx = (double *)take(N * D); I.Dxy = (double *)take(N * D);
+  I.Dyy = (double *)take(N * D); I.itau = (double *)take(N * D);
   I.f = (double *)take(N * D); I.R = (double *)take(N * D); I.x = (double *)take(N * D);
   for (int b = 0; b < 5; ++b) I.band64[b] = (double *)take(N * D);
   I.r64 = mixed ? (double *)take(N * D) : nullptr;
@@ -77,7 +77,7 @@ static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   I.wy0 = (double *)take((ny + 1) * D); I.wy1 = (double *)take((ny + 1) * D);
   I.bcv[0] = (double *)take((ny + 1) * D); I.bcv[1] = (double *)take((ny + 1) * D);
   I.bcv[2] = (double *)take((nx + 1) * D); I.bcv[3] = (double *)take((nx + 1) * D);
-  I.partial = (double *)take((size_t)4 * I.nblk0 * D);
+  I.partial = (double *)take((size_t)kNumPartials * I.nblk0 * D);
   // PC32 keeps its shadow residual and its search direction in FP32 (PV_RH,
   // PV_P) instead of VRH and VP
   for (int v = 0; v < kNumVec; ++v)
@@ -198,10 +198,22 @@ struct DevMaps {   // offsets into one device buffer
   int maxlev;
 };
 
-static const char *kKernelNames[] = {"assemble",   "rap",        "line_l0",    "line_coarse",
-                                     "restrict",   "prolong",    "spmv_v",     "spmv_t",
-                                     "vec_update", "ir_residual", "finalize"};
-constexpr int kNumKernelIds = 11;
+// Kernel families timed by sg_batch_time_kernel (HIP events around every
+// dispatch of the family): the assembly; the per-step set-up (prolongation
+// weights, Galerkin products, line factorisations); the level-0 line sweeps;
+// the line sweeps of coarser levels (and the exact solves of the coarsest);
+// restriction; prolongation; the two SpMVs of BiCGSTAB; its vector updates;
+// the FP64 residual of iterative refinement; the step-end finalize and
+// commit; the fused coarse cycle; the per-instance reduction finishes.
+static const char *kKernelNames[] = {"assemble",    "rap",         "line_l0",    "line_coarse",
+                                     "restrict",    "prolong",     "spmv_v",     "spmv_t",
+                                     "vec_update",  "ir_residual", "finalize",   "coarse_cycle",
+                                     "finish"};
+constexpr int kNumKernelIds = 13;
+constexpr int kTimeAll = kNumKernelIds;   // sg_batch_time_kernel id: every family
+// Krylov iterations the per-step graph holds at most (more are run in
+// pipelined chunks after it)
+constexpr int kHeadIters = 16;
 
 struct Batch {
   sg_opts opts{};
@@ -224,44 +236,58 @@ struct Batch {
   int *h_cnt = nullptr;         // pinned
   int *h_pipe = nullptr;        // pinned, counters of the pipelined Krylov chunks
   hipEvent_t ev_cnt[2] = {nullptr, nullptr};
-  // Krylov chunks captured as graphs: key (path, xstate, iterations) -> executable.
-  // The graphs hold the map and instance pointers and launch sizes of the
-  // current batch layout, so every rebuild drops them.
+  // Captured graphs by key: the per-step head (set-up, Krylov start, the first
+  // iterations and a step-end pass) and chunks of iterations. They hold the map
+  // and instance pointers and launch sizes of the current batch layout, so
+  // every rebuild drops them.
   struct ChunkGraph { int key; hipGraphExec_t exec; };
   std::vector<ChunkGraph> graphs;
   void drop_graphs();
+  int last_iters_max = 0;   // largest Krylov iteration count of the last step
   void *d_maps = nullptr;
   size_t d_maps_bytes = 0;
   DevMaps dm{};
   std::vector<int> order;       // compact index -> slot
   std::vector<int> part_host;
   bool dirty = true;
-  double *d_scratch = nullptr;  // layout transposes
+  double *d_scratch = nullptr;  // layout transposes and coefficient staging
   size_t scratch_elems = 0;
-  std::vector<KScal> h_scal;
-  // kernel timing
-  int time_id = -1;
+  KScal *h_scal = nullptr;      // pinned, by slot (capacity entries)
+  // kernel timing: bit k of time_mask times family k
+  int time_mask = 0;
   std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_ids;      // family of each recorded pair
   size_t ev_used = 0;
-  double t_ms = 0;
-  int64_t t_launches = 0;
-  double t_bytes = 0;   // algorithmic bytes of the timed launches
+  double t_ms[kNumKernelIds] = {};
+  int64_t t_launches[kNumKernelIds] = {};
+  double t_bytes[kNumKernelIds] = {};   // algorithmic bytes of the timed launches
+  void reset_time() {
+    for (int k = 0; k < kNumKernelIds; ++k) { t_ms[k] = 0; t_launches[k] = 0; t_bytes[k] = 0; }
+  }
 
   ~Batch();
   int init(const sg_opts *o, size_t budget);
   int ensure_capacity(int cap);
   int add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const double *f0, double dt,
           int *slot);
-  int upload_coeffs(Slot &s, const sg_coeffs *c);
+  int set_coeffs(Slot &s, const sg_coeffs *c);
   int upload_bc(Slot &s, int side, const double *vals);
   int remove(int slot);
   int rebuild();
   int fuse_level(const InstDev &I) const;
   int step(int nsteps, const int64_t *targets, int ntargets);
   int step_once();
+  bool graphs_on() const { return opts.use_graph >= 0 && time_mask == 0; }
+  // Runs what enqueue() enqueues on `stream`: replayed from the graph captured
+  // under `key` (captured on first use), or launched eagerly.
+  template <typename F> int run_graph(int key, F &&enqueue);
+  void setup_kernels(bool start64);
+  void end_pass();
   // KT: Krylov vectors, VT: V-cycle (KT double with VT float is SG_PC32)
-  template <typename KT, typename VT> int krylov(int maxit, bool xstate);
+  template <typename KT, typename VT> int krylov(int maxit, bool xstate, int it0, bool ends);
+  template <typename KT, typename VT> int solve64();
   template <typename KT, typename VT> void iteration(bool xstate);
+  void count_bytes(const std::vector<int> &part);
   template <typename T> void vcycle(int l, int xid, int bid);
   template <typename F> void fan_out(const int (&cnt)[kNumTileShapes], F &&launch);
   int read_counter(int which, int *out);
@@ -283,6 +309,7 @@ Batch::~Batch() {
   (void)hipFree(d_maps); (void)hipFree(d_scratch);
   if (h_cnt) (void)hipHostFree(h_cnt);
   if (h_pipe) (void)hipHostFree(h_pipe);
+  if (h_scal) (void)hipHostFree(h_scal);
   drop_graphs();
   for (auto ev : ev_cnt)
     if (ev) (void)hipEventDestroy(ev);
@@ -354,15 +381,18 @@ int Batch::ensure_capacity(int cap) {
     HIPCHK(hipMemcpyAsync(nt, d_ticket, sizeof(unsigned) * 2 * capacity, hipMemcpyDeviceToDevice,
                           stream));
   }
+  KScal *nh = nullptr;
+  HIPCHK(hipHostMalloc((void **)&nh, sizeof(KScal) * nc, hipHostMallocDefault));
+  std::memset(nh, 0, sizeof(KScal) * nc);
   HIPCHK(hipStreamSynchronize(stream));
   (void)hipFree(d_scal); (void)hipFree(d_ticket); (void)hipFree(d_insts); (void)hipFree(d_part);
-  d_scal = ns; d_ticket = nt; d_insts = ni; d_part = np;
+  if (h_scal) (void)hipHostFree(h_scal);
+  d_scal = ns; d_ticket = nt; d_insts = ni; d_part = np; h_scal = nh;
   capacity = nc;
   for (size_t k = 0; k < slots.size(); ++k) {
     slots[k].dev.sc = d_scal + k;
     slots[k].dev.ticket = d_ticket + 2 * k;
   }
-  h_scal.resize(nc);
   dirty = true;
   return SG_OK;
 }
@@ -384,8 +414,13 @@ static int check_grid(const sg_grid *g) {
   return SG_OK;
 }
 
-static int check_coeffs(const sg_coeffs *c) {
-  if (!c || !c->G || !c->Dxx || !c->Dxy || !c->Dyy) return fail(SG_ERR_ARG, "coeffs: null pointer");
+// Creation needs G and D; inv_tau may be NULL (1/tau = 0,
+// source/Equation.h:39-40). A refresh (sg_set_coeffs) takes any subset: NULL
+// arrays keep their device values.
+static int check_coeffs(const sg_coeffs *c, bool create) {
+  if (!c) return fail(SG_ERR_ARG, "coeffs: null pointer");
+  if (create && (!c->G || !c->Dxx || !c->Dxy || !c->Dyy))
+    return fail(SG_ERR_ARG, "coeffs: null pointer");
   return SG_OK;
 }
 
@@ -405,31 +440,36 @@ static void vertex_weights(const double *e, int n, std::vector<double> &w0, std:
   }
 }
 
-int Batch::upload_coeffs(Slot &s, const sg_coeffs *c) {
-  const int nx = s.nx, ny = s.ny;
-  const size_t N = (size_t)nx * ny;
-  std::vector<double> h(5 * N);
-  double *G = h.data(), *Lxx = G + N, *Lxy = Lxx + N, *Lyy = Lxy + N, *it = Lyy + N;
-  for (int i = 0; i < nx; ++i)
-    for (int j = 0; j < ny; ++j) {
-      const size_t q = (size_t)i * ny + j, K = cell_slot(i, j, nx);
-      const double g = c->G[q];
-      if (!std::isfinite(g) || !std::isfinite(c->Dxx[q]) || !std::isfinite(c->Dxy[q]) ||
-          !std::isfinite(c->Dyy[q]))
-        return fail(SG_ERR_ARG, "coeffs: non-finite G or D at cell (" + std::to_string(i) + "," +
-                                    std::to_string(j) + ")");
-      G[K] = g;
-      // Lambda = G * D (source/Solver.cc:61-62)
-      Lxx[K] = c->Dxx[q] * g;
-      Lxy[K] = c->Dxy[q] * g;
-      Lyy[K] = c->Dyy[q] * g;
-      it[K] = c->inv_tau ? c->inv_tau[q] : 0.0;
-    }
-  HIPCHK(hipMemcpyAsync((void *)s.dev.G, G, N * 8, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemcpyAsync((void *)s.dev.Lxx, Lxx, N * 8, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemcpyAsync((void *)s.dev.Lxy, Lxy, N * 8, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemcpyAsync((void *)s.dev.Lyy, Lyy, N * 8, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemcpyAsync((void *)s.dev.itau, it, N * 8, hipMemcpyHostToDevice, stream));
+// The given arrays of one instance, in the (nx, ny) host layout, replace the
+// device ones (device cell order). They are staged on the device and checked
+// there for non-finite values before any of them is written, so a rejected
+// call changes nothing; the layout change runs on the device, and the
+// assembly forms Lambda = G * D (source/Solver.cc:57-65) as it reads them, so
+// a refresh after Equation::update (source/Solver.cc:287-288) moves only the
+// arrays that changed and runs no per-cell host loop.
+int Batch::set_coeffs(Slot &s, const sg_coeffs *c) {
+  const double *src[5] = {c->G, c->Dxx, c->Dxy, c->Dyy, c->inv_tau};
+  double *dst[5] = {(double *)s.dev.G, (double *)s.dev.Dxx, (double *)s.dev.Dxy,
+                    (double *)s.dev.Dyy, (double *)s.dev.itau};
+  const size_t N = (size_t)s.nx * s.ny;
+  int nsrc = 0;
+  for (auto p : src) nsrc += p != nullptr;
+  if (!nsrc) return SG_OK;
+  int rc;
+  if ((rc = scratch((size_t)nsrc * N))) return rc;
+  for (int k = 0, q = 0; k < 5; ++k)
+    if (src[k])
+      HIPCHK(hipMemcpyAsync(d_scratch + (size_t)(q++) * N, src[k], N * 8, hipMemcpyHostToDevice,
+                            stream));
+  launch_count_nonfinite(stream, d_scratch, (long long)nsrc * N, d_cnt);
+  HIPCHK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (h_cnt[0])
+    return fail(SG_ERR_ARG, "coeffs: " + std::to_string(h_cnt[0]) +
+                                " non-finite values of G, D or 1/tau");
+  for (int k = 0, q = 0; k < 5; ++k)
+    if (src[k]) launch_to_korder(stream, d_scratch + (size_t)(q++) * N, dst[k], s.nx, s.ny);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   return SG_OK;
 }
@@ -448,7 +488,7 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
                int *slot_out) {
   int rc;
   if ((rc = check_grid(g))) return rc;
-  if ((rc = check_coeffs(c))) return rc;
+  if ((rc = check_coeffs(c, true))) return rc;
   if (!bc || !f0 || !slot_out) return fail(SG_ERR_ARG, "bc, f0 and slot must not be null");
   if (!(dt > 0) || !std::isfinite(dt)) return fail(SG_ERR_ARG, "dt must be positive");
   for (int s = 0; s < 4; ++s) {
@@ -480,6 +520,7 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
   s.nx = g->nx; s.ny = g->ny; s.prec = opts.precision; s.slab = slab; s.bytes = bytes; s.dt = dt;
   carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, opts.precision, s.dev);
   s.dev.dt = dt;
+  s.dev.noclamp = opts.no_clamp ? 1 : 0;
   s.dev.sc = d_scal + slot;
   s.dev.ticket = d_ticket + 2 * slot;
   s.dev.bcmask = 0;
@@ -502,7 +543,9 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
   vertex_weights(g->y_edges, ny, w0, w1);
   (void)hipMemcpy((void *)s.dev.wy0, w0.data(), (ny + 1) * 8, hipMemcpyHostToDevice);
   (void)hipMemcpy((void *)s.dev.wy1, w1.data(), (ny + 1) * 8, hipMemcpyHostToDevice);
-  if ((rc = upload_coeffs(s, c))) return undo(rc);
+  if (!c->inv_tau && hipMemsetAsync((void *)s.dev.itau, 0, N * 8, stream) != hipSuccess)
+    return undo(fail(SG_ERR_HIP, "clearing 1/tau failed"));
+  if ((rc = set_coeffs(s, c))) return undo(rc);
   for (int k = 0; k < 4; ++k)
     if (bc->type[k] == SG_DIRICHLET && (rc = upload_bc(s, k, bc->values[k]))) return undo(rc);
   std::vector<double> fk(N);
@@ -705,7 +748,7 @@ int Batch::read_counter(int which, int *out) {
 }
 
 void Batch::tbegin(int id, hipStream_t s) {
-  if (id != time_id) return;
+  if (!(time_mask >> id & 1)) return;
   if (!s) s = stream;
   if (ev_used + 2 > ev_pool.size()) {
     for (int k = 0; k < 64; ++k) {
@@ -717,19 +760,22 @@ void Batch::tbegin(int id, hipStream_t s) {
 }
 
 void Batch::tend(int id, hipStream_t s) {
-  if (id != time_id || ev_used + 2 > ev_pool.size()) return;
+  if (!(time_mask >> id & 1) || ev_used + 2 > ev_pool.size()) return;
   (void)hipEventRecord(ev_pool[ev_used + 1], s ? s : stream);
+  if (ev_ids.size() < ev_pool.size() / 2) ev_ids.resize(ev_pool.size() / 2);
+  ev_ids[ev_used / 2] = id;
   ev_used += 2;
 }
 
 int Batch::collect_time() {
   if (!ev_used) return SG_OK;
-  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipDeviceSynchronize());   // the side stream's events too
   for (size_t k = 0; k + 1 < ev_used; k += 2) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ev_pool[k], ev_pool[k + 1]));
-    t_ms += ms;
-    t_launches += 1;
+    const int id = ev_ids[k / 2];
+    t_ms[id] += ms;
+    t_launches[id] += 1;
   }
   ev_used = 0;
   return SG_OK;
@@ -785,17 +831,27 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   // first level and no per-level prolongation has started: all of them run
   // here, in one launch, one workgroup per instance.
   const bool deeper = l + 1 < m.maxlev && m.nrmap[l + 1] > 0;
-  if (!deeper) launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm);
+  if (!deeper && m.nfmap) {
+    tbegin(11);
+    launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm);
+    tend(11);
+  }
   if (pre_x) sweep(0);
   if (pre_y) sweep(1);
   const bool pre_zero = zero;
   // exact solve on the coarsest level of the instances that end here
-  tbegin(3);
   for (int sh = 0; sh < kNumTileShapes; ++sh) {
-    launch_line<T>(stream, d_insts, m.swx[l][sh], m.nswx[l][sh], l, 0, sh, true, xid, bid);
-    launch_line<T>(stream, d_insts, m.swy[l][sh], m.nswy[l][sh], l, 1, sh, true, xid, bid);
+    if (m.nswx[l][sh]) {
+      tbegin(3);
+      launch_line<T>(stream, d_insts, m.swx[l][sh], m.nswx[l][sh], l, 0, sh, true, xid, bid);
+      tend(3);
+    }
+    if (m.nswy[l][sh]) {
+      tbegin(3);
+      launch_line<T>(stream, d_insts, m.swy[l][sh], m.nswy[l][sh], l, 1, sh, true, xid, bid);
+      tend(3);
+    }
   }
-  tend(3);
   if (deeper) {
     tbegin(4);
     if (!pre_zero) launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, xid, bid);
@@ -819,55 +875,139 @@ template <typename KT, typename VT> void Batch::iteration(bool xstate) {
   tbegin(8); launch_bicg<KT>(stream, 0, d_insts, m0, nb, xstate, pc); tend(8);
   vcycle<VT>(0, pc ? PV_PH : VPH, pc ? PV_P : VP);
   tbegin(6); launch_bicg<KT>(stream, 1, d_insts, m0, nb, xstate, pc); tend(6);
-  launch_finish<KT>(stream, FIN_V, d_insts, n, d_cnt);
+  tbegin(12); launch_finish<KT>(stream, FIN_V, d_insts, n, d_cnt); tend(12);
   tbegin(8); launch_bicg<KT>(stream, 2, d_insts, m0, nb, xstate, pc); tend(8);
-  launch_finish<KT>(stream, FIN_S, d_insts, n, d_cnt);
+  tbegin(12); launch_finish<KT>(stream, FIN_S, d_insts, n, d_cnt); tend(12);
   vcycle<VT>(0, pc ? PV_SH : VSH, pc ? PV_B : VS);
   tbegin(7); launch_bicg<KT>(stream, 3, d_insts, m0, nb, xstate, pc); tend(7);
-  launch_finish<KT>(stream, FIN_T, d_insts, n, d_cnt);
+  tbegin(12); launch_finish<KT>(stream, FIN_T, d_insts, n, d_cnt); tend(12);
   tbegin(8); launch_bicg<KT>(stream, 4, d_insts, m0, nb, xstate, pc); tend(8);
-  launch_finish<KT>(stream, FIN_X, d_insts, n, d_cnt);
+  tbegin(12); launch_finish<KT>(stream, FIN_X, d_insts, n, d_cnt); tend(12);
 }
 
-// Krylov iterations in chunks of check_every. The next chunk is enqueued
-// before the host waits for the active-instance counter of the current one, so
-// the GPU never drains while the host decides; a chunk enqueued after the
-// last instance converged costs only early-exiting launches.
-template <typename KT, typename VT> int Batch::krylov(int maxit, bool xstate) {
+template <typename F> int Batch::run_graph(int key, F &&enqueue) {
+  if (!graphs_on()) {
+    enqueue();
+    HIPCHK(fan_err);
+    return SG_OK;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto &g : graphs)
+    if (g.key == key) exec = g.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    const hipError_t ce = hipStreamEndCapture(stream, &graph);
+    HIPCHK(fan_err);
+    HIPCHK(ce);
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(ie);
+    graphs.push_back({key, exec});
+  }
+  HIPCHK(hipGraphLaunch(exec, stream));
+  return SG_OK;
+}
+
+// The set-up of a step: statuses and counters, assembly, the Galerkin chain of
+// the multigrid hierarchy and the line factorisations; start64: also the start
+// of the FP64 BiCGSTAB (the assembly wrote r0 and its partial sums).
+void Batch::setup_kernels(bool start64) {
+  const int n = (int)order.size(), nb0 = dm.ncmap[0];
+  const bool mixed = opts.precision == SG_MIXED;
+  const bool h32 = opts.precision != SG_FP64;   // FP32 hierarchy and V-cycle
+  const double tol2 = mixed ? opts.inner_rtol * opts.inner_rtol : opts.rtol * opts.rtol;
+  launch_begin(stream, d_insts, n, d_part, d_cnt, d_cnt + 1, tol2, opts.rtol * opts.rtol,
+               mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
+  tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
+  // LU factors of the line matrices of level l: smoothing lines of both
+  // colours and the exact solve on the coarsest level
+  auto factors = [&](int l, hipStream_t s) {
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p)
+        for (int sh = 0; sh < kNumTileShapes; ++sh)
+          launch_line_factor(s, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d, sh, h32);
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      launch_line_factor(s, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
+      launch_line_factor(s, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
+    }
+  };
+  auto note = [&](hipError_t e) { if (e != hipSuccess && fan_err == hipSuccess) fan_err = e; };
+  if (side && time_mask == 0) {
+    // A level's factorisations depend only on its operator, and nothing else
+    // in the set-up reads them: they run on the side stream, each level's as
+    // soon as its operator exists, beside the chain of prolongation weights and
+    // Galerkin products that builds the next level on the main stream. The
+    // main stream waits for them before the solve. Same kernels, same inputs.
+    note(hipEventRecord(ev_lev[0], stream));
+    for (int l = 0; l + 1 < dm.maxlev; ++l) {
+      launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
+      launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
+      note(hipEventRecord(ev_lev[l + 1], stream));
+    }
+    for (int l = 0; l < dm.maxlev; ++l) {
+      note(hipStreamWaitEvent(side, ev_lev[l], 0));
+      factors(l, side);
+    }
+    note(hipEventRecord(ev_join, side));
+    note(hipStreamWaitEvent(stream, ev_join, 0));
+  } else {
+    for (int l = 0; l < dm.maxlev; ++l) {
+      tbegin(1);
+      factors(l, stream);
+      if (l + 1 < dm.maxlev) {
+        launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
+        launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
+      }
+      tend(1);
+    }
+  }
+  if (start64) {
+    tbegin(12); launch_finish<double>(stream, FIN_INIT64, d_insts, n, d_cnt); tend(12);
+  }
+}
+
+// A step-end pass over the instances whose solve ended since the previous
+// pass: true residual, counts and min of x, the acceptance decision, and f of
+// the accepted ones. Instances finalized by an earlier pass are left alone, so
+// passes may follow every chunk of iterations.
+void Batch::end_pass() {
+  const int n = (int)order.size(), nb0 = dm.ncmap[0];
+  tbegin(10); launch_finalize(stream, d_insts, dm.cmap[0], nb0); tend(10);
+  tbegin(12); launch_finish<double>(stream, FIN_FINALIZE, d_insts, n, d_cnt); tend(12);
+  tbegin(10); launch_commit(stream, d_insts, dm.cmap[0], nb0); tend(10);
+}
+
+static int graph_key(int kind, int n) { return (kind << 16) | n; }
+
+// Krylov iterations from iteration it0, in chunks of check_every (each chunk
+// followed by a step-end pass when `ends`), until the active-instance counter
+// reads 0 or maxit. The next chunk is enqueued before the host waits for the
+// counter of the current one, so the GPU never drains while the host decides;
+// a chunk enqueued after the last instance converged costs only early-exiting
+// launches.
+template <typename KT, typename VT>
+int Batch::krylov(int maxit, bool xstate, int it0, bool ends) {
   if (!ev_cnt[0]) {
     for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&ev_cnt[k], hipEventDisableTiming));
   }
+  const int variant = (sizeof(KT) == 4 ? 0 : sizeof(VT) == 4 ? 1 : 2) * 4 + (xstate ? 2 : 0) +
+                      (ends ? 1 : 0);
   auto chunk = [&](int slot, int &it) -> int {
     const int c = std::min(opts.check_every, maxit - it);
-    if (opts.use_graph >= 0 && time_id < 0) {
-      // replay the captured chunk (kernel timing records host-side events,
-      // so a timed run launches eagerly)
-      const int key = ((sizeof(KT) == 4 ? 0 : sizeof(VT) == 4 ? 1 : 2) * 2 + (xstate ? 1 : 0)) * 4096 + c;
-      hipGraphExec_t exec = nullptr;
-      for (auto &g : graphs)
-        if (g.key == key) exec = g.exec;
-      if (!exec) {
-        hipGraph_t graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
-        HIPCHK(hipStreamEndCapture(stream, &graph));
-        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPCHK(ie);
-        graphs.push_back({key, exec});
-      }
-      HIPCHK(hipGraphLaunch(exec, stream));
-    } else {
+    int rc = run_graph(graph_key(2 + variant, c), [&] {
       for (int k = 0; k < c; ++k) iteration<KT, VT>(xstate);
-    }
+      if (ends) end_pass();
+    });
+    if (rc) return rc;
     it += c;
-    HIPCHK(fan_err);
     HIPCHK(hipMemcpyAsync(h_pipe + slot, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev_cnt[slot], stream));
     return SG_OK;
   };
-  int it = 0, slot = 0, rc;
-  if (maxit <= 0) return SG_OK;
+  int it = it0, slot = 0, rc;
+  if (it >= maxit) return SG_OK;
   if ((rc = chunk(slot, it))) return rc;
   for (;;) {
     const bool more = it < maxit;
@@ -877,6 +1017,83 @@ template <typename KT, typename VT> int Batch::krylov(int maxit, bool xstate) {
     slot ^= 1;
   }
   return SG_OK;
+}
+
+// FP64 BiCGSTAB (FP64 or FP32 V-cycle). The head of the step -- set-up, the
+// Krylov start, as many iterations as the previous step needed (at most
+// kHeadIters) and a step-end pass -- is one graph launch; the host then reads
+// the active-instance counter once. Only when some instance still iterates do
+// pipelined chunks of iterations, each with its own step-end pass, follow.
+template <typename KT, typename VT> int Batch::solve64() {
+  const int maxit = opts.max_iters;
+  const int k0 = std::max(0, std::min({last_iters_max, maxit, kHeadIters}));
+  int rc = run_graph(graph_key(1, k0), [&] {
+    setup_kernels(true);
+    for (int k = 0; k < k0; ++k) iteration<KT, VT>(true);
+    end_pass();
+  });
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(h_pipe, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (h_pipe[0] == 0 || k0 >= maxit) return SG_OK;
+  return krylov<KT, VT>(maxit, true, k0, true);
+}
+
+// Algorithmic bytes of the timed families in one step, from the V-cycles,
+// iterations and refinements each participating instance ran: every array a
+// kernel reads or writes counted once per launch, T the V-cycle precision.
+void Batch::count_bytes(const std::vector<int> &part) {
+  const int prec = opts.precision;
+  const double T = prec == SG_FP64 ? 8.0 : 4.0;   // hierarchy / V-cycle values
+  const double h32 = prec != SG_FP64 ? 1.0 : 0.0;
+  // BiCGSTAB per cell and iteration (SpMV v, SpMV t, and the p, s, x updates)
+  const double bv = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 4 + 8 : 64.0;
+  const double bt = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 8 + 8 : 64.0;
+  const double bvec = prec == SG_MIXED ? 60.0 : prec == SG_PC32 ? 24.0 + 28.0 + 52.0 : 120.0;
+  const int sm = opts.smoother;
+  const bool post_x = sm & 4;
+  for (size_t c = 0; c < part.size(); ++c) {
+    if (!part[c]) continue;
+    const Slot &s = slots[order[c]];
+    const KScal &k = h_scal[order[c]];
+    const InstDev &I = s.dev;
+    const double N0 = (double)s.nx * s.ny, vc = k.vcyc, it = k.titers;
+    t_bytes[0] += N0 * (48.0 + 56.0 + 20.0 * h32 + (prec == SG_MIXED ? 0.0 : 8.0) +
+                        (prec == SG_FP64 ? 24.0 : 0.0));
+    for (int l = 0; l < I.nlev; ++l) {
+      const double Nl = (double)I.lev[l].nx * I.lev[l].ny;
+      const double nb = l == 0 ? 5.0 : 9.0;
+      // line factorisations: the 5 bands they read, 2 factors per direction written
+      t_bytes[1] += Nl * (5.0 * 8.0 + 4.0 * T);
+      if (l + 1 < I.nlev) {
+        const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
+        // prolongation weights (bands in, 4 weights out) and the Galerkin
+        // product (bands and weights in, 9 FP64 bands and their copies out)
+        t_bytes[1] += Nl * (nb * 8.0 + 4.0 * T) + Nl * (nb * 8.0 + 4.0 * T) +
+                      Nc * (72.0 + 36.0 * h32);
+      }
+      if (l >= s.fuse_lev) continue;   // the fused coarse cycle
+      if (l == I.nlev - 1) {           // exact line solve of the coarsest level
+        t_bytes[3] += vc * Nl * 5.0 * T;
+        continue;
+      }
+      // a V-cycle sweeps every cell once per direction: 8 (5-point) or 12
+      // (9-point) values per swept cell
+      t_bytes[l == 0 ? 2 : 3] += vc * 2.0 * (l == 0 ? 8.0 : 12.0) * T * Nl;
+      const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
+      // restriction of b into level l + 1 (x = 0): r and the 4 weights per
+      // fine cell, b out
+      t_bytes[4] += vc * (5.0 * T * Nl + T * Nc);
+      // prolongation into the lines the first sweep reads (half the cells):
+      // their weights, x in and out, and the coarse x once
+      t_bytes[5] += vc * ((post_x ? 0.5 : 1.0) * Nl * 5.0 * T + T * Nc);
+    }
+    t_bytes[6] += it * bv * N0;
+    t_bytes[7] += it * bt * N0;
+    t_bytes[8] += it * bvec * N0;
+    if (prec == SG_MIXED) t_bytes[9] += (k.orefs + 1) * 64.0 * N0;
+    t_bytes[10] += N0 * (56.0 + 16.0);
+  }
 }
 
 int Batch::step_once() {
@@ -897,111 +1114,59 @@ int Batch::step_once() {
   }
   const bool mixed = opts.precision == SG_MIXED;
   const bool h32 = opts.precision != SG_FP64;   // FP32 hierarchy and V-cycle
-  const double tol2 = mixed ? opts.inner_rtol * opts.inner_rtol : opts.rtol * opts.rtol;
-  launch_begin(stream, d_insts, n, d_part, d_cnt, d_cnt + 1, tol2, opts.rtol * opts.rtol,
-               mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
-  const int nb0 = dm.ncmap[0];
-  tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
-  // LU factors of the line matrices of level l: smoothing lines of both
-  // colours and the exact solve on the coarsest level
-  auto factors = [&](int l, hipStream_t s) {
-    for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p)
-        for (int sh = 0; sh < kNumTileShapes; ++sh)
-          launch_line_factor(s, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d, sh, h32);
-    for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      launch_line_factor(s, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
-      launch_line_factor(s, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
-    }
-  };
-  if (side && time_id < 0) {
-    // A level's factorisations depend only on its operator, and nothing else
-    // in the set-up reads them: they run on the side stream, each level's as
-    // soon as its operator exists, beside the chain of prolongation weights and
-    // Galerkin products that builds the next level on the main stream. The
-    // main stream waits for them before the solve. Same kernels, same inputs.
-    HIPCHK(hipEventRecord(ev_lev[0], stream));
-    for (int l = 0; l + 1 < dm.maxlev; ++l) {
-      launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
-      launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
-      HIPCHK(hipEventRecord(ev_lev[l + 1], stream));
-    }
-    for (int l = 0; l < dm.maxlev; ++l) {
-      HIPCHK(hipStreamWaitEvent(side, ev_lev[l], 0));
-      factors(l, side);
-    }
-    HIPCHK(hipEventRecord(ev_join, side));
-    HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
-  } else {
-    for (int l = 0; l < dm.maxlev; ++l) {
-      tbegin(1);
-      factors(l, stream);
-      if (l + 1 < dm.maxlev) {
-        launch_pweights(stream, d_insts, dm.pmap[l], dm.npmap[l], l, h32);
-        launch_rap(stream, d_insts, dm.cmap[l + 1], dm.ncmap[l + 1], l, h32);
-      }
-      tend(1);
-    }
-  }
   if (!mixed) {
-    // the assembly started BiCGSTAB (r = rh = R - M f, p = v = 0, partials)
-    launch_finish<double>(stream, FIN_INIT64, d_insts, n, d_cnt);
-    if ((rc = h32 ? krylov<double, float>(opts.max_iters, true)
-                  : krylov<double, double>(opts.max_iters, true)))
-      return rc;
+    if ((rc = h32 ? solve64<double, float>() : solve64<double, double>())) return rc;
   } else {
+    if ((rc = run_graph(graph_key(1, 0), [&] { setup_kernels(false); }))) return rc;
+    const int nb0 = dm.ncmap[0];
     for (int ref = 0; ref <= opts.max_iters; ++ref) {
       tbegin(9); launch_ir_residual(stream, d_insts, dm.cmap[0], nb0); tend(9);
-      launch_finish<double>(stream, FIN_IR_RES, d_insts, n, d_cnt + 1);
+      tbegin(12); launch_finish<double>(stream, FIN_IR_RES, d_insts, n, d_cnt + 1); tend(12);
       int oactive = 0;
       if ((rc = read_counter(1, &oactive))) return rc;
       if (!oactive) break;
       launch_ir_inner_init(stream, d_insts, dm.cmap[0], nb0);
-      launch_finish<float>(stream, FIN_INNER_INIT, d_insts, n, d_cnt);
+      tbegin(12); launch_finish<float>(stream, FIN_INNER_INIT, d_insts, n, d_cnt); tend(12);
       launch_copy_counter(stream, d_cnt + 1, d_cnt);
-      if ((rc = krylov<float, float>(opts.max_inner, false))) return rc;
+      if ((rc = krylov<float, float>(opts.max_inner, false, 0, false))) return rc;
       launch_ir_update(stream, d_insts, dm.cmap[0], nb0);
     }
+    end_pass();
   }
-  tbegin(10); launch_finalize(stream, d_insts, dm.cmap[0], nb0); tend(10);
-  launch_finish<double>(stream, FIN_FINALIZE, d_insts, n, d_cnt);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h_scal.data(), d_scal, sizeof(KScal) * slots.size(), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(h_scal, d_scal, sizeof(KScal) * slots.size(), hipMemcpyDeviceToHost,
                         stream));
   HIPCHK(hipStreamSynchronize(stream));
-  if ((rc = collect_time())) return rc;
-  if (time_id == 2) {
-    // Algorithmic traffic of the timed level-0 sweeps, from the V-cycles each
-    // instance ran (a converged instance's work-groups exit at once): per
-    // swept cell b, the lower band, the two line factors and the two off-line
-    // couplings are read, one value of x of the other colour is read (each is
-    // shared by the two lines beside it) and x is written: 8 values. A V-cycle
-    // sweeps every cell twice (rows and columns, each in two colours).
-    const double vt = opts.precision == SG_FP64 ? sizeof(double) : sizeof(float);
-    for (int c = 0; c < n; ++c) {
-      if (!part[c]) continue;
-      const Slot &s = slots[order[c]];
-      if (s.fuse_lev == 0) continue;   // its level 0 runs in the fused coarse cycle
-      t_bytes += h_scal[order[c]].vcyc * 2.0 * 8.0 * vt * (double)s.nx * s.ny;
-    }
+  if (time_mask) {
+    if ((rc = collect_time())) return rc;
+    count_bytes(part);
   }
-  int worst = SG_OK;
+  int worst = SG_OK, nfail = 0, maxit = 0;
   for (int c = 0; c < n; ++c) {
     if (!part[c]) continue;
     Slot &s = slots[order[c]];
     const KScal &k = h_scal[order[c]];
-    s.steps += 1;
+    const bool ok = k.ostatus == ST_DONE;
+    if (ok) s.steps += 1;   // a rejected step leaves f and the step count as they were
+    maxit = std::max(maxit, k.iters);
     s.st.steps = s.steps;
     s.st.last_iters = k.titers;
     s.st.total_iters += k.titers;
     s.st.last_refinements = mixed ? k.orefs : 0;
     s.st.last_rel_residual = k.true_rel;
-    s.st.min_f = k.minf;
-    s.st.clamp_count += k.clamps;
-    const bool failed = mixed ? (k.ostatus == ST_FAIL) : (k.status == ST_FAIL);
-    s.st.status = failed ? SG_ERR_NOCONV : SG_OK;
-    if (failed) worst = SG_ERR_NOCONV;
+    s.st.last_min_x = k.minf;
+    s.st.min_f = (opts.no_clamp || !ok) ? k.minf : std::max(k.minf, 0.0);
+    s.st.last_clamps = ok ? k.clamps : 0;
+    s.st.clamp_count += s.st.last_clamps;
+    s.st.last_nonfinite = k.nonfin;
+    s.st.status = ok ? SG_OK : SG_ERR_NOCONV;
+    if (!ok) { worst = SG_ERR_NOCONV; ++nfail; }
   }
+  last_iters_max = maxit;
+  if (nfail)
+    g_err = "step rejected for " + std::to_string(nfail) +
+            " instance(s): the solve did not reach its tolerance, broke down or left non-finite "
+            "values (f and the step count are unchanged)";
   return worst;
 }
 
@@ -1014,7 +1179,7 @@ int Batch::step(int nsteps, const int64_t *targets, int ntargets) {
   int worst = SG_OK;
   for (int s = 0; s < nsteps; ++s) {
     rc = step_once();
-    if (rc == SG_ERR_NOCONV) { worst = rc; g_err = "Krylov solve did not converge"; continue; }
+    if (rc == SG_ERR_NOCONV) { worst = rc; continue; }
     if (rc) return rc;
   }
   return worst;
@@ -1112,7 +1277,7 @@ struct sg_batch {
   size_t budget = 0;
   int device = 0;
   std::vector<std::pair<int, int>> where;   // global slot -> (group, local slot); group -1: free
-  int time_id = -1;
+  int time_mask = 0;
 
   Batch *group_of(int slot, int *local) {
     if (slot < 0 || slot >= (int)where.size() || where[slot].first < 0) return nullptr;
@@ -1126,7 +1291,7 @@ struct sg_batch {
     if (inited[k]) return SG_OK;
     int rc = g[k].init(&opts, 0);
     if (rc) return rc;
-    g[k].time_id = time_id;
+    g[k].time_mask = time_mask;
     inited[k] = true;
     return SG_OK;
   }
@@ -1279,6 +1444,15 @@ int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values
   if (!(s.dev.bcmask & (1 << side))) return fail(SG_ERR_ARG, "side is not Dirichlet");
   return G->upload_bc(s, side, values);
 }
+int sg_batch_set_coeffs(sg_batch *b, int slot, const sg_coeffs *c) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  int rc = sg::check_coeffs(c, false);
+  if (rc) return rc;
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  return G->set_coeffs(G->slots[local], c);
+}
 int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out) {
   if (!b || !out) return fail(SG_ERR_ARG, "null argument");
   int local;
@@ -1293,26 +1467,45 @@ void sg_batch_destroy(sg_batch *b) { delete b; }
 
 int sg_batch_time_kernel(sg_batch *b, int kernel_id) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  if (kernel_id >= sg::kNumKernelIds) return fail(SG_ERR_ARG, "unknown kernel id");
-  b->time_id = kernel_id;
+  if (kernel_id > sg::kTimeAll) return fail(SG_ERR_ARG, "unknown kernel id");
+  const int mask = kernel_id < 0 ? 0
+                   : kernel_id == sg::kTimeAll ? (1 << sg::kNumKernelIds) - 1
+                                               : 1 << kernel_id;
+  b->time_mask = mask;
   for (auto &G : b->g) {
-    G.time_id = kernel_id;
-    G.t_ms = 0;
-    G.t_launches = 0;
-    G.t_bytes = 0;
+    G.time_mask = mask;
+    G.reset_time();
   }
   return SG_OK;
 }
 int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, double *bytes) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
-  if (total_ms) *total_ms = b->g[0].t_ms + b->g[1].t_ms;
-  if (launches) *launches = b->g[0].t_launches + b->g[1].t_launches;
-  if (bytes) *bytes = b->g[0].t_bytes + b->g[1].t_bytes;
+  double ms = 0, by = 0;
+  int64_t nl = 0;
+  for (auto &G : b->g)
+    for (int k = 0; k < sg::kNumKernelIds; ++k) {
+      ms += G.t_ms[k];
+      nl += G.t_launches[k];
+      by += G.t_bytes[k];
+    }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = nl;
+  if (bytes) *bytes = by;
+  return SG_OK;
+}
+int sg_batch_kernel_time_id(sg_batch *b, int id, double *total_ms, int64_t *launches,
+                            double *bytes) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  if (id < 0 || id >= sg::kNumKernelIds) return fail(SG_ERR_ARG, "unknown kernel id");
+  if (total_ms) *total_ms = b->g[0].t_ms[id] + b->g[1].t_ms[id];
+  if (launches) *launches = b->g[0].t_launches[id] + b->g[1].t_launches[id];
+  if (bytes) *bytes = b->g[0].t_bytes[id] + b->g[1].t_bytes[id];
   return SG_OK;
 }
 const char *sg_kernel_name(int id) {
   return (id >= 0 && id < sg::kNumKernelIds) ? sg::kKernelNames[id] : "";
 }
+int sg_kernel_count(void) { return sg::kNumKernelIds; }
 
 int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, const double *f0,
               double dt, const sg_opts *opts, sg_solver **out) {
@@ -1333,12 +1526,7 @@ int sg_set_bc_values(sg_solver *h, int side, const double *values) {
 }
 int sg_set_coeffs(sg_solver *h, const sg_coeffs *c) {
   if (!h) return fail(SG_ERR_ARG, "null handle");
-  int rc = sg::check_coeffs(c);
-  if (rc) return rc;
-  int local;
-  Batch *G = h->bt->group_of(h->slot, &local);
-  if (!G) return fail(SG_ERR_ARG, "no instance");
-  return G->upload_coeffs(G->slots[local], c);
+  return sg_batch_set_coeffs(h->bt, h->slot, c);
 }
 int sg_step(sg_solver *h, int nsteps) {
   if (!h) return fail(SG_ERR_ARG, "null handle");

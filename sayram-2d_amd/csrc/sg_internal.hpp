@@ -22,6 +22,8 @@
 namespace sg {
 
 constexpr int kMaxLev = 16;
+// partial sums per level-0 workgroup (the finalize pass uses all five)
+constexpr int kNumPartials = 5;
 // Cells per instance: every per-cell array then stays below 2 GiB, so kernels
 // may address it with 32-bit byte offsets from its base.
 constexpr long long kMaxCells = 1LL << 28;
@@ -109,15 +111,20 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   double onorm2, obnorm2, otol2;   // mixed: outer (FP64) residual norms
   double oprev2;         // mixed: residual norm^2 of the previous refinement
   double best2;          // smallest Krylov residual norm^2 of this solve
-  int best_it, pad2;     // iteration at which it was reached
-  double true_rel;       // finalize: ||R - M f|| / ||R||
-  double minf;           // finalize: min f before clamping
+  int best_it;           // iteration at which it was reached
+  int phase;             // step end: 0 solving, 1 finalized in this pass (f is
+                         // written next), 2 finished in an earlier pass
+  double true_rel;       // finalize: ||R - M x|| / ||R|| of the solution x
+  double minf;           // finalize: min x (f before clamping)
   int iters, status;     // Krylov iteration count and status of this solve
-  int ostatus, orefs;    // mixed: outer refinement status and count
+  int ostatus, orefs;    // mixed: outer refinement status and count; every
+                         // precision: ST_FAIL after finalize rejects the step
   int maxit, clamps;
   int omaxit, part;
   int titers;           // Krylov iterations summed over the refinements of a step
   int vcyc;             // level-0 V-cycles this instance ran in the step
+  int nonfin;           // finalize: cells whose x is not finite
+  int pad3;
 };
 
 struct Lev {            // one multigrid level of one instance
@@ -134,21 +141,24 @@ struct InstDev {
   int nx, ny, nlev, bcmask;   // bcmask bit s set: side s Dirichlet
   int nblk0;                  // level-0 workgroups of this instance
   int prec;                   // sg_precision of the instance
+  int noclamp;                // 1: f = x at the end of a step (no clamp at 0)
+  int pad;
   double dt;
   // geometry (source/Mesh.cc:36-65) and per-vertex interpolation weights of
   // fill_vertex_from_cells (source/Solver.cc:326-381), computed once
   const double *xe, *ye;
   const double *wx0, *wx1, *wy0, *wy1;
   const double *bcv[4];
-  // Equation inputs, device cell order: G, Lambda = G*D (source/Solver.cc:57-65), 1/tau
-  const double *G, *Lxx, *Lxy, *Lyy, *itau;
+  // Equation inputs, device cell order: G, D, 1/tau (the assembly forms
+  // Lambda = G*D, source/Solver.cc:57-65, as it reads them)
+  const double *G, *Dxx, *Dxy, *Dyy, *itau;
   // FP64 state
   double *f, *R, *x, *r64;
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
   float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle inputs (s, p) and outputs,
                                      // shadow residual
-  double *partial;            // 4 * nblk0
+  double *partial;            // kNumPartials * nblk0
   unsigned int *ticket;       // 2 tickets
   KScal *sc;
   Lev lev[kMaxLev];

@@ -13,6 +13,7 @@
 #include "sg_kernels.hpp"
 #include "sayram_gpu.h"   // sg_precision values of InstDev::prec
 
+#include <algorithm>
 #include <cfloat>
 
 namespace sg {
@@ -142,6 +143,7 @@ __global__ void k_begin(const InstDev *__restrict__ insts, int n, const int *__r
     s.ostatus = p ? ST_ACTIVE : ST_IDLE;
     s.tol2 = tol2; s.otol2 = otol2; s.maxit = maxit; s.omaxit = omaxit;
     s.iters = 0; s.titers = 0; s.orefs = 0; s.clamps = 0; s.vcyc = 0;
+    s.phase = 0; s.nonfin = 0;
     mine += p;
   }
   atomicAdd(&cnt, mine);
@@ -311,18 +313,23 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
     const double f3[3][3] = {{f[rm + sm], f[rm + s0], f[rm + sp]},
                              {f[r0 + sm], f[K], f[r0 + sp]},
                              {f[rp + sm], f[rp + s0], f[rp + sp]}};
-    const auto Lxx = gbl(I.Lxx), Lxy = gbl(I.Lxy), Lyy = gbl(I.Lyy);
+    // Lambda = D * G per cell, entry by entry as update_Lambda forms it
+    // (source/Solver.cc:57-65)
+    const auto Dxx = gbl(I.Dxx), Dxy = gbl(I.Dxy), Dyy = gbl(I.Dyy), Gv = gbl(I.G);
     const size_t kw = r0 + sm, ke = r0 + sp, ks = rm + s0, kn = rp + s0;
-    const double LK[3] = {Lxx[K], Lxy[K], Lyy[K]};
-    const double LW[3] = {Lxx[kw], Lxy[kw], Lyy[kw]}, LE[3] = {Lxx[ke], Lxy[ke], Lyy[ke]};
-    const double LS[3] = {Lxx[ks], Lxy[ks], Lyy[ks]}, LN[3] = {Lxx[kn], Lxy[kn], Lyy[kn]};
+    auto lam = [&](size_t q, double (&L)[3]) {
+      const double g = Gv[q];
+      L[0] = Dxx[q] * g; L[1] = Dxy[q] * g; L[2] = Dyy[q] * g;
+    };
+    double LK[3], LW[3], LE[3], LS[3], LN[3];
+    lam(K, LK); lam(kw, LW); lam(ke, LE); lam(ks, LS); lam(kn, LN);
     const auto xe = gbl(I.xe), ye = gbl(I.ye);
     const double xm = xe[hw ? i - 1 : i], x0 = xe[i], x1 = xe[i + 1], x2 = xe[he ? i + 2 : i + 1];
     const double ym = ye[hs ? j - 1 : j], y0 = ye[j], y1 = ye[j + 1], y2 = ye[hn ? j + 2 : j + 1];
     const auto wx0 = gbl(I.wx0), wx1 = gbl(I.wx1), wy0 = gbl(I.wy0), wy1 = gbl(I.wy1);
     const double wxLi = wx0[i], wxRi = wx1[i], wxLe = wx0[i + 1], wxRe = wx1[i + 1];
     const double wyBj = wy0[j], wyTj = wy1[j], wyBn = wy0[j + 1], wyTn = wy1[j + 1];
-    const double GK = gbl(I.G)[K], itK = gbl(I.itau)[K];
+    const double GK = Gv[K], itK = gbl(I.itau)[K];
     const CellEdges cK{x0, x1, y0, y1}, cW{xm, x0, y0, y1}, cE{x1, x2, y0, y1};
     const CellEdges cS{x0, x1, ym, y0}, cN{x0, x1, y1, y2};
     const double vSW = vertex_value(I, i, j, f3[0][0], f3[0][1], f3[1][0], f3[1][1], wxLi, wxRi,
@@ -1678,16 +1685,27 @@ __global__ void __launch_bounds__(kBlock) k_ir_update(const InstDev *__restrict_
   gbl(I.x)[K] += rsc(I)->scale * (double)gbl(ptr<float>(I.kv[VXK]))[K];
 }
 
-// f = max(x, 0); partials of the true residual of the unclamped x, ||R||, the
-// clamp count and the minimum (slot 3).
+// Whether an instance's solve of this step has ended: the FP64 refinement
+// (mixed precision), or the Krylov solve (the other precisions).
+__device__ __forceinline__ bool solve_ended(int prec, int status, int ostatus) {
+  if (prec == SG_MIXED) return ostatus == ST_DONE || ostatus == ST_FAIL;
+  return status == ST_DONE || status == ST_FAIL;
+}
+
+// Finalize pass over the instances whose solve has ended and that were not
+// finalized in an earlier pass of this step (phase 0): partials of the true
+// residual ||R - M x||, ||R||, the count of negative x (the cells f = max(x, 0)
+// clamps), the count of non-finite x and min x (slot 4). The finish
+// kernel decides from them whether the step is accepted; k_commit then writes
+// f. Nothing here writes f.
 __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (rsc(I)->ostatus == ST_IDLE) return;
+  if (rsc(I)->phase != 0 || !solve_ended(I.prec, rsc(I)->status, rsc(I)->ostatus)) return;
   const int K = m.k0 + threadIdx.x;
   const int N = I.nx * I.ny;
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   double mn = DBL_MAX;
   if (K < N) {
     int i, j;
@@ -1697,22 +1715,40 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
     v[0] = r * r;
     v[1] = gbl(I.R)[K] * gbl(I.R)[K];
     v[2] = xv < 0.0 ? 1.0 : 0.0;
+    v[3] = isfinite(xv) ? 0.0 : 1.0;
     mn = xv;
   }
   __shared__ double smin[kBlock / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mn = fmin(mn, __shfl_down(mn, off, 64));
   if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = mn;
-  store_partials<double, 3>(I, m.k0 / kBlock, v);   // contains __syncthreads
+  block_sum<double, 4>(v);   // contains __syncthreads
   if (threadIdx.x == 0) {
     double b = smin[0];
     for (int w = 1; w < kBlock / 64; ++w) b = fmin(b, smin[w]);
-    I.partial[(size_t)3 * I.nblk0 + m.k0 / kBlock] = b;
+    const int lblk = m.k0 / kBlock;
+    I.partial[lblk] = v[0];
+    I.partial[(size_t)I.nblk0 + lblk] = v[1];
+    I.partial[(size_t)2 * I.nblk0 + lblk] = v[2];
+    I.partial[(size_t)3 * I.nblk0 + lblk] = v[3];
+    I.partial[(size_t)4 * I.nblk0 + lblk] = b;
   }
-  if (K < N) {
-    const double xv = gbl(I.x)[K];
-    gbl(I.f)[K] = xv > 0.0 ? xv : 0.0;   // min f >= 0 enforced exactly
-  }
+}
+
+// f = max(x, 0) (or f = x with noclamp) for the instances finalized in this
+// pass whose step was accepted; min f >= 0 is then exact. A rejected step
+// (ostatus ST_FAIL: the solve broke down, hit its iteration cap, or left a
+// residual above the accepted floor, or x has non-finite cells) leaves f as it
+// was.
+__global__ void __launch_bounds__(kBlock) k_commit(const InstDev *__restrict__ insts,
+                                                  const BlkMap *__restrict__ map) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (rsc(I)->phase != 1 || rsc(I)->ostatus != ST_DONE) return;
+  const int K = m.k0 + threadIdx.x;
+  if (K >= I.nx * I.ny) return;
+  const double xv = gbl(I.x)[K];
+  gbl(I.f)[K] = (I.noclamp || xv > 0.0) ? xv : 0.0;
 }
 
 // Finish of a reduction: one workgroup per instance sums the partial sums in a
@@ -1761,7 +1797,11 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
   if (MODE == F_IR_RES || MODE == F_INNER_INIT) {
     if (ost != ST_ACTIVE) return;
   } else if (MODE == F_FINALIZE) {
-    if (ost == ST_IDLE) return;
+    if (s.phase == 1) {   // committed by the previous pass
+      if (threadIdx.x == 0) s.phase = 2;
+      return;
+    }
+    if (s.phase != 0 || !solve_ended(I.prec, st, ost)) return;
   } else if (MODE == F_X) {
     if (st != ST_ACTIVE && st != ST_HALF) return;
   } else {
@@ -1782,18 +1822,44 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
   } else if (MODE == F_V) {   // after the V-cycle of p (active instances only)
     T t[1];
     sum_partials<T, 1>(I, t);
-    if (threadIdx.x == 0) { s.alpha = (double)((T)s.rho / t[0]); s.vcyc += 1; }
+    if (threadIdx.x == 0) {
+      s.vcyc += 1;
+      // Breakdown: (rh, v) zero or not finite (a V-cycle output that
+      // overflowed). The solve stops with x at its last iterate, which no
+      // later kernel of the step updates (they skip failed instances).
+      const double alpha = (double)((T)s.rho / t[0]);
+      if (t[0] == T(0) || !isfinite((double)t[0]) || !isfinite(alpha)) {
+        s.status = ST_FAIL;
+        atomicSub(cnt, 1);
+      } else {
+        s.alpha = alpha;
+      }
+    }
   } else if (MODE == F_S) {
     T t[1];
     sum_partials<T, 1>(I, t);
-    if (threadIdx.x == 0 && (double)t[0] <= s.tol2 * s.bnorm2) {
-      s.status = ST_HALF;
-      s.rnorm2 = (double)t[0];
+    if (threadIdx.x == 0) {
+      if (!isfinite((double)t[0])) {
+        s.status = ST_FAIL;
+        atomicSub(cnt, 1);
+      } else if ((double)t[0] <= s.tol2 * s.bnorm2) {
+        s.status = ST_HALF;
+        s.rnorm2 = (double)t[0];
+      }
     }
   } else if (MODE == F_T) {   // after the V-cycle of s (still-active instances only)
     T t[2];
     sum_partials<T, 2>(I, t);
-    if (threadIdx.x == 0) { s.omega = t[1] > T(0) ? (double)(t[0] / t[1]) : 0.0; s.vcyc += 1; }
+    if (threadIdx.x == 0) {
+      s.vcyc += 1;
+      const double omega = t[1] > T(0) ? (double)(t[0] / t[1]) : 0.0;
+      if (!isfinite((double)t[0]) || !isfinite((double)t[1]) || !isfinite(omega)) {
+        s.status = ST_FAIL;   // before x moves along a non-finite direction
+        atomicSub(cnt, 1);
+      } else {
+        s.omega = omega;
+      }
+    }
   } else if (MODE == F_X) {
     T t[2] = {T(0), T(0)};
     if (st == ST_ACTIVE) sum_partials<T, 2>(I, t);
@@ -1841,7 +1907,11 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       const bool floor = !first && t[0] > 0.25 * s.oprev2 &&
                          t[0] <= fmax(1e-18, 1e4 * s.otol2) * s.obnorm2;
       s.oprev2 = t[0];
-      if (conv || floor) {
+      if (!isfinite(t[0]) || !isfinite(t[1])) {
+        s.ostatus = ST_FAIL;
+        s.status = ST_DONE;
+        atomicSub(cnt, 1);
+      } else if (conv || floor) {
         s.ostatus = ST_DONE;
         s.status = ST_DONE;
         atomicSub(cnt, 1);
@@ -1864,11 +1934,11 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       s.status = ST_ACTIVE;
     }
   } else {   // F_FINALIZE
-    double t[3];
-    sum_partials<double, 3>(I, t);
+    double t[4];
+    sum_partials<double, 4>(I, t);
     __shared__ double sm[kFinThreads];
     double mm = DBL_MAX;
-    const double *pmin = I.partial + (size_t)3 * I.nblk0;
+    const double *pmin = I.partial + (size_t)4 * I.nblk0;
     int b = threadIdx.x;
     for (; b + 7 * kFinThreads < I.nblk0; b += 8 * kFinThreads) {
       double v[8];
@@ -1885,9 +1955,18 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       __syncthreads();
     }
     if (threadIdx.x == 0) {
+      const double nf = t[3];
       s.true_rel = t[1] > 0.0 ? sqrt(t[0] / t[1]) : sqrt(t[0]);
-      s.clamps = (int)t[2];
+      s.nonfin = (int)nf;
+      s.clamps = (I.noclamp || nf > 0.0) ? 0 : (int)t[2];
       s.minf = sm[0];
+      // The step is accepted when the solve ended without failing, x is
+      // finite and the true residual meets the tolerance, or the rounding
+      // floor the stall rule accepts (1e-9 relative) if that is larger.
+      const bool failed = I.prec == SG_MIXED ? ost == ST_FAIL : st == ST_FAIL;
+      const bool res_ok = t[0] <= fmax(s.otol2, 1e-18) * t[1];
+      s.ostatus = (!failed && nf == 0.0 && res_ok) ? ST_DONE : ST_FAIL;
+      s.phase = 1;
     }
   }
 }
@@ -1901,6 +1980,22 @@ __global__ void __launch_bounds__(kBlock) k_to_korder(const double *__restrict__
   const int j = (int)(K / nx), i = xcol((int)(K - (long long)j * nx), half_x(nx));
   dst[K] = src[(size_t)i * ny + j];
 }
+// Count of the non-finite values of src[0 .. n) added to *cnt (validation of
+// host arrays uploaded to a staging buffer before they replace device state).
+__global__ void __launch_bounds__(kBlock) k_count_nonfinite(const double *__restrict__ src,
+                                                           long long n, int *__restrict__ cnt) {
+  int bad = 0;
+  for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < n;
+       q += (long long)gridDim.x * kBlock)
+    bad += isfinite(src[q]) ? 0 : 1;
+  int v[1] = {bad};
+  block_sum<int, 1>(v);
+  if (threadIdx.x == 0 && v[0]) atomicAdd(cnt, v[0]);
+}
+__global__ void k_zero_int(int *__restrict__ p, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_to_imajor(const double *__restrict__ src,
                                                      double *__restrict__ dst, int nx, int ny) {
   const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -2086,6 +2181,14 @@ void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) 
 }
 void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_finalize<<<nb, kBlock, 0, s>>>(d, m);
+}
+void launch_commit(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
+  if (nb) k_commit<<<nb, kBlock, 0, s>>>(d, m);
+}
+void launch_count_nonfinite(hipStream_t s, const double *src, long long n, int *cnt) {
+  k_zero_int<<<1, 64, 0, s>>>(cnt, 1);
+  const long long nb = std::min<long long>(nblocks(n, kBlock), 2048);
+  if (n > 0) k_count_nonfinite<<<(unsigned)nb, kBlock, 0, s>>>(src, n, cnt);
 }
 void launch_to_korder(hipStream_t s, const double *src, double *dst, int nx, int ny) {
   k_to_korder<<<nblocks((long long)nx * ny, kBlock), kBlock, 0, s>>>(src, dst, nx, ny);

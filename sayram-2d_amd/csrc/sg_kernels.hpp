@@ -45,7 +45,12 @@ void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb
 void launch_ir_inner_init(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_copy_counter(hipStream_t s, const int *src, int *dst);
 void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+// finalize pass over the instances whose solve ended (true residual, clamp
+// and non-finite counts, min x); k_commit writes f of the accepted ones
 void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+void launch_commit(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+// *cnt = number of non-finite values in src[0 .. n)
+void launch_count_nonfinite(hipStream_t s, const double *src, long long n, int *cnt);
 void launch_to_korder(hipStream_t s, const double *src, double *dst, int nx, int ny);
 void launch_to_imajor(hipStream_t s, const double *src, double *dst, int nx, int ny);
 

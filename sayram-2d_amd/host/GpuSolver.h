@@ -9,19 +9,25 @@
 //   t()                 istep * dt                source/Solver.h:23
 //   f(), f(ind)         the state, (nx, ny)       source/Solver.h:24-25
 //
-// It borrows the Mesh and the Equation like Solver does. Per-cell G, D and 1/tau
-// are gathered once through the Equation accessors (Lambda = G*D is formed in the
-// library); the Dirichlet vertex values are gathered at t = 0 and again after
-// every step at the new t, as fill_vertex_from_bcs does
-// (source/Solver.cc:385-422), calling Equation::update(t) first
-// (source/Solver.cc:287). Equations whose G or D change in update(t) set
-// coeffs_time_dependent so that they are re-uploaded every step.
+// It borrows the Mesh and the Equation like Solver does. After every step it
+// calls Equation::update(t) (source/Solver.cc:287), gathers the per-cell G, D
+// and 1/tau again through the Equation accessors, as update_Lambda does
+// (source/Solver.cc:57-65, 288), and hands the library only the arrays that
+// changed (Lambda = G*D is formed on the device); the Dirichlet vertex values
+// are gathered at t = 0 and after every step at the new t, as
+// fill_vertex_from_bcs does (source/Solver.cc:385-422). An Equation known to
+// keep its coefficients fixed may pass coeffs_constant = true to skip the
+// per-step gather.
 //
 // Errors become std::runtime_error, as in the reference (source/Solver.cc:393).
+// A step the library rejects (SG_ERR_NOCONV: the solve did not converge, broke
+// down or left non-finite values; f and the step count are unchanged) throws
+// too: the reference's direct solve never yields an unconverged step.
 #pragma once
 
 #include <array>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -37,8 +43,8 @@ template <class MeshT, class EquationT, class ArrayT = SAYRAM_GPU_ARRAY_DEFAULT>
 class GpuSolver {
  public:
   GpuSolver(const MeshT &m_in, EquationT *eqp, const sg_opts *opts = nullptr,
-            bool coeffs_time_dependent = false)
-      : m(m_in), eq(*eqp), time_dependent_(coeffs_time_dependent) {
+            bool coeffs_constant = false)
+      : m(m_in), eq(*eqp), constant_(coeffs_constant) {
     nx_ = m.nx();
     ny_ = m.ny();
     std::vector<double> xe(nx_ + 1), ye(ny_ + 1);
@@ -64,15 +70,28 @@ class GpuSolver {
 
   void update() {
     const int rc = sg_step(h_, 1);
-    if (rc != SG_OK && rc != SG_ERR_NOCONV) check(rc);
     last_status_ = rc;
+    check(rc);
     istep_ += 1;
     stale_ = true;
     eq.update(t());
-    if (time_dependent_) {
+    if (!constant_) {
+      // only the arrays whose values changed cross to the device
+      std::vector<double> *now[5] = {&G_, &Dxx_, &Dxy_, &Dyy_, &itau_};
+      for (int k = 0; k < 5; ++k) prev_[k].swap(*now[k]);
       gather_coeffs();
-      sg_coeffs c{G_.data(), Dxx_.data(), Dxy_.data(), Dyy_.data(), itau_.data()};
-      check(sg_set_coeffs(h_, &c));
+      const double *p[5];
+      bool any = false;
+      for (int k = 0; k < 5; ++k) {
+        const bool same = prev_[k].size() == now[k]->size() &&
+                          std::memcmp(prev_[k].data(), now[k]->data(), prev_[k].size() * 8) == 0;
+        p[k] = same ? nullptr : now[k]->data();
+        any |= !same;
+      }
+      if (any) {
+        sg_coeffs c{p[0], p[1], p[2], p[3], p[4]};
+        check(sg_set_coeffs(h_, &c));
+      }
     }
     sg_bc bc{};
     gather_bc(t(), bc);
@@ -106,12 +125,13 @@ class GpuSolver {
  private:
   const MeshT &m;
   EquationT &eq;
-  bool time_dependent_;
+  bool constant_;
   std::size_t nx_ = 0, ny_ = 0;
   std::int64_t istep_ = 0;
   sg_solver *h_ = nullptr;
   int last_status_ = SG_OK;
   std::vector<double> G_, Dxx_, Dxy_, Dyy_, itau_;
+  std::vector<double> prev_[5];   // the arrays of the previous gather
   std::array<std::vector<double>, 4> bcv_;
   mutable ArrayT f_;
   mutable std::vector<double> host_;

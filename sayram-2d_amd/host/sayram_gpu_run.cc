@@ -1,7 +1,12 @@
 // sayram_gpu_run -- the reference's driver (source/main.cc) on the gfx950 step.
 //
-//   sayram_gpu_run [--case albert_young|albert_young_lc] [--dump-case <file>] <p.ini>
+//   sayram_gpu_run [--case albert_young|albert_young_lc] [--dump-case <file>] [<p.ini>]
 //   sayram_gpu_run [--dump-case <file>] <p.ini> <problem.bin>
+// Solver options: --precision 0|1|2 (sg_precision), --rtol x, --max-iters n.
+//
+// With no parameter file it reads p.ini of the working directory
+// (source/Parameters.cc:20-23), and it copies the parameter file to
+// ./output/<run_id>/<run_id>.ini (source/Parameters.cc:10-16).
 //
 // Without problem.bin the Equation is the reference's Albert & Young chorus
 // case (source/main.cc:54 constructs Albert_Young; albert_young_lc is the
@@ -29,15 +34,16 @@
 #include <cctype>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <string>
-#include <sys/stat.h>
 #include <vector>
 
 #include "GpuSolver.h"
@@ -116,8 +122,14 @@ Params read_params(const std::string &path) {
   p.nplots = std::stoi(get("diagnostics", "nplots"));
   auto dc = ini.find("diffusion_coefficients");
   if (dc != ini.end() && dc->second.count("did")) p.dID = dc->second.at("did");
-  // source/Parameters.cc:59-60
+  // source/Parameters.cc:59-60. With nsteps < nplots the reference rounds
+  // nsteps to 0 and dt = T / 0: there is no step to take, so that is an error.
+  if (p.nplots < 1) throw std::runtime_error("[diagnostics] nplots must be >= 1");
   p.save_every = (int)(p.nsteps / p.nplots);
+  if (p.save_every < 1)
+    throw std::runtime_error("nsteps (" + get("basic", "nsteps") + ") is less than nplots (" +
+                             std::to_string(p.nplots) +
+                             "): nsteps rounds down to a multiple of nplots, here 0 steps");
   p.nsteps = p.save_every * p.nplots;
   return p;
 }
@@ -163,15 +175,34 @@ class ArrayCase : public Equation {
       rd(val_[s], s < 2 ? ny_ + 1 : nx_ + 1);
     }
     if (!in) throw std::runtime_error("truncated problem file");
+    // optional: rates of a time-dependent Equation (sayram_cases.time_dependent)
+    in.read(magic, 8);
+    if (in.gcount() == 8) {
+      if (std::memcmp(magic, "SGTIME1", 7) != 0) throw std::runtime_error("bad problem file trailer");
+      in.read(reinterpret_cast<char *>(rate_), sizeof(rate_));
+      if (!in) throw std::runtime_error("truncated problem file");
+      timed_ = true;
+      base_[0] = G_; base_[1] = Dxx_; base_[2] = Dxy_; base_[3] = Dyy_; base_[4] = inv_tau_;
+    }
   }
   BCType bc_type(BoundaryID side) const override { return type_[(int)side]; }
   double init_f(const Ind &ind) const override { return f0_(ind.i, ind.j); }
-  void update(double) override {}
-  bool dirichlet_vertex_value(BoundaryID side, std::size_t i, std::size_t j, double,
+  // At time t: G (1 + rG t), D (1 + rD t), 1/tau (1 + rtau t) of the base
+  // arrays; the Dirichlet values scale by (1 + rbc t) in dirichlet_vertex_value.
+  void update(double t) override {
+    if (!timed_) return;
+    const double sG = 1.0 + rate_[0] * t, sD = 1.0 + rate_[1] * t, sT = 1.0 + rate_[2] * t;
+    Array2D *dst[5] = {&G_, &Dxx_, &Dxy_, &Dyy_, &inv_tau_};
+    const double sc[5] = {sG, sD, sD, sD, sT};
+    for (int a = 0; a < 5; ++a)
+      for (std::size_t q = 0; q < nx_ * ny_; ++q) dst[a]->data()[q] = base_[a].data()[q] * sc[a];
+  }
+  bool dirichlet_vertex_value(BoundaryID side, std::size_t i, std::size_t j, double t,
                               double *out) const override {
     const int s = (int)side;
     if (type_[s] != BCType::Dirichlet) return false;
-    *out = val_[s][s < 2 ? j : i];
+    const double v = val_[s][s < 2 ? j : i];
+    *out = timed_ ? v * (1.0 + rate_[3] * t) : v;
     return true;
   }
 
@@ -180,6 +211,9 @@ class ArrayCase : public Equation {
   Array2D f0_;
   BCType type_[4];
   std::vector<double> val_[4];
+  bool timed_ = false;
+  double rate_[4] = {0, 0, 0, 0};
+  Array2D base_[5];
 };
 
 #ifdef SAYRAM_HDF5
@@ -278,20 +312,34 @@ void dump_case(const Mesh &m, const Equation &eq, const std::string &path) {
 int main(int argc, char **argv) {
   std::string case_name = "albert_young", dump;
   std::vector<std::string> pos;
+  sg_opts opts{};   // zero: the library defaults
   for (int k = 1; k < argc; ++k) {
     const std::string a = argv[k];
     if (a == "--case" && k + 1 < argc) case_name = argv[++k];
     else if (a == "--dump-case" && k + 1 < argc) dump = argv[++k];
+    else if (a == "--precision" && k + 1 < argc) opts.precision = std::atoi(argv[++k]);
+    else if (a == "--rtol" && k + 1 < argc) opts.rtol = std::atof(argv[++k]);
+    else if (a == "--max-iters" && k + 1 < argc) opts.max_iters = std::atoi(argv[++k]);
     else pos.push_back(a);
   }
-  if (pos.empty() || pos.size() > 2 ||
-      (case_name != "albert_young" && case_name != "albert_young_lc")) {
+  // no parameter file: p.ini of the working directory (source/Parameters.cc:20-23)
+  if (pos.empty()) pos.push_back("p.ini");
+  if (pos.size() > 2 || (case_name != "albert_young" && case_name != "albert_young_lc")) {
     std::cerr << "usage: sayram_gpu_run [--case albert_young|albert_young_lc] [--dump-case <file>] "
-                 "<p.ini> [<problem.bin>]" << std::endl;
+                 "[--precision p] [--rtol x] [--max-iters n] [<p.ini> [<problem.bin>]]"
+              << std::endl;
     return 1;
   }
   try {
     const Params paras = read_params(pos[0]);
+    // the output directory and a copy of the parameter file in it, as the
+    // reference's Parameters constructor makes them (source/Parameters.cc:7-16)
+    const std::string out_dir = "./output/" + paras.run_id + "/";
+    if (dump.empty()) {
+      std::filesystem::create_directories(out_dir);
+      std::filesystem::copy_file(pos[0], out_dir + paras.run_id + ".ini",
+                                 std::filesystem::copy_options::overwrite_existing);
+    }
     Grid2D grid = make_uniform(paras);
     Mesh m(grid, paras.dt());
     std::unique_ptr<Equation> eqp;
@@ -311,10 +359,7 @@ int main(int argc, char **argv) {
       dump_case(m, eq, dump);
       return 0;
     }
-    GpuSolver solver(m, &eq);
-    const std::string out_dir = "./output/" + paras.run_id + "/";
-    ::mkdir("./output", 0755);
-    ::mkdir(out_dir.c_str(), 0755);
+    GpuSolver solver(m, &eq, &opts);
     const std::size_t nx = m.nx(), ny = m.ny();
 #ifdef SAYRAM_HDF5
     const std::string fname = out_dir + paras.run_id + "_data.h5";

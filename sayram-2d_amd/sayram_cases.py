@@ -21,6 +21,7 @@ Every cell array is returned in the reference's ``Xtensor2d`` layout: shape
 """
 from __future__ import annotations
 
+import copy
 import math
 from dataclasses import dataclass, field
 
@@ -129,11 +130,26 @@ def batch_specs(count: int, lo: int, hi: int, seed: int):
     return out
 
 
+def batch_problems(count: int = 512, lo: int = 32, hi: int = 512, big: int = 4096,
+                   seed: int = 2026):
+    """The batched workloads: `count` instances of batch_specs, plus one big x
+    big grid (D seed derived from `seed`, amplitude 1) when big > 0. With the
+    defaults: 512 instances with dims uniform in 32..512 and one 4096^2 grid,
+    55.0 M cells."""
+    probs = [seeded_case(s["nx"], s["ny"], seed=s["seed"], scale=s["scale"])
+             for s in batch_specs(count, lo, hi, seed)]
+    if big:
+        probs.append(seeded_case(big, big, seed=seed * 7 + 1, scale=1.0))
+    return probs
+
+
 def write_problem(p: Problem, path: str) -> None:
     """Binary problem file read by the C++ driver (host/sayram_gpu_run.cc):
     magic 'SGPROB1\\0', int32 nx, ny, then G, Dxx, Dxy, Dyy, inv_tau, f0 as
     (nx, ny) float64 arrays, int32 bc types[4], and the four Dirichlet vertex
-    arrays (ny+1, ny+1, nx+1, nx+1 values; zeros for zero-flux sides)."""
+    arrays (ny+1, ny+1, nx+1, nx+1 values; zeros for zero-flux sides). A
+    time-dependent problem (p.meta["rates"], see time_dependent) appends magic
+    'SGTIME1\\0' and its four rates as float64."""
     with open(path, "wb") as fh:
         fh.write(b"SGPROB1\0")
         fh.write(np.array([p.nx, p.ny], dtype=np.int32).tobytes())
@@ -143,6 +159,35 @@ def write_problem(p: Problem, path: str) -> None:
         for s, n in zip(range(4), (p.ny + 1, p.ny + 1, p.nx + 1, p.nx + 1)):
             v = p.bc_val[s] if p.bc_val[s] is not None else np.zeros(n)
             fh.write(np.ascontiguousarray(v, dtype=np.float64).tobytes())
+        if p.meta.get("rates") is not None:
+            fh.write(b"SGTIME1\0")
+            fh.write(np.array(p.meta["rates"], dtype=np.float64).tobytes())
+
+
+def time_dependent(p: Problem, rG: float = 0.0, rD: float = 0.0, rtau: float = 0.0,
+                   rbc: float = 0.0) -> Problem:
+    """Marks p time-dependent: at time t the Equation holds G (1 + rG t),
+    D (1 + rD t), 1/tau (1 + rtau t) and Dirichlet values (1 + rbc t) of the
+    base arrays (linear in t, so the C++ driver's Equation and problem_at
+    round every value identically)."""
+    q = copy.copy(p)
+    q.meta = dict(p.meta, rates=(float(rG), float(rD), float(rtau), float(rbc)))
+    return q
+
+
+def problem_at(p: Problem, t: float) -> Problem:
+    """The inputs a time-dependent problem's Equation holds after update(t)
+    (source/Solver.cc:287-289): the step from t uses them."""
+    rates = p.meta.get("rates")
+    if rates is None:
+        return p
+    rG, rD, rtau, rbc = rates
+    q = copy.copy(p)
+    q.G = p.G * (1.0 + rG * t)
+    q.Dxx, q.Dxy, q.Dyy = (a * (1.0 + rD * t) for a in (p.Dxx, p.Dxy, p.Dyy))
+    q.inv_tau = p.inv_tau * (1.0 + rtau * t)
+    q.bc_val = [None if v is None else v * (1.0 + rbc * t) for v in p.bc_val]
+    return q
 
 
 # ---------------------------------------------------------------------------

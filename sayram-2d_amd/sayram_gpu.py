@@ -41,13 +41,14 @@ class SgOpts(C.Structure):
                 ("inner_rtol", C.c_double), ("max_inner", C.c_int), ("check_every", C.c_int),
                 ("smoother", C.c_int), ("use_graph", C.c_int), ("line_tiles", C.c_int * 4),
                 ("fuse_cells", C.c_int), ("streams", C.c_int),
-                ("group_cells", C.c_int), ("reserved", C.c_int * 1)]
+                ("group_cells", C.c_int), ("no_clamp", C.c_int)]
 
 
 class SgStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("last_iters", C.c_int), ("last_refinements", C.c_int),
                 ("last_rel_residual", C.c_double), ("min_f", C.c_double),
-                ("clamp_count", C.c_int64), ("total_iters", C.c_int64), ("status", C.c_int)]
+                ("clamp_count", C.c_int64), ("total_iters", C.c_int64), ("status", C.c_int),
+                ("last_nonfinite", C.c_int), ("last_clamps", C.c_int64), ("last_min_x", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -79,6 +80,7 @@ SYMBOLS = [
     ("sg_batch_get_f", C.c_int, [_V, C.c_int, _dp]),
     ("sg_batch_set_f", C.c_int, [_V, C.c_int, _dp, C.c_int64]),
     ("sg_batch_set_bc_values", C.c_int, [_V, C.c_int, C.c_int, _dp]),
+    ("sg_batch_set_coeffs", C.c_int, [_V, C.c_int, C.POINTER(SgCoeffs)]),
     ("sg_batch_stats", C.c_int, [_V, C.c_int, C.POINTER(SgStats)]),
     ("sg_batch_size", C.c_int, [_V]),
     ("sg_batch_bytes_in_use", _S, [_V]),
@@ -87,13 +89,17 @@ SYMBOLS = [
     ("sg_batch_time_kernel", C.c_int, [_V, C.c_int]),
     ("sg_batch_kernel_time", C.c_int, [_V, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                        C.POINTER(C.c_double)]),
+    ("sg_batch_kernel_time_id", C.c_int, [_V, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_double)]),
     ("sg_kernel_name", C.c_char_p, [C.c_int]),
+    ("sg_kernel_count", C.c_int, []),
     ("sg_solver_batch", C.c_int, [_V, C.POINTER(_V)]),
 ]
 
 KERNEL_IDS = {"assemble": 0, "rap": 1, "line_l0": 2, "line_coarse": 3, "restrict": 4,
               "prolong": 5, "spmv_v": 6, "spmv_t": 7, "vec_update": 8, "ir_residual": 9,
-              "finalize": 10}
+              "finalize": 10, "coarse_cycle": 11, "finish": 12}
+TIME_ALL = len(KERNEL_IDS)   # sg_batch_time_kernel id that times every family
 
 
 class SayramError(RuntimeError):
@@ -133,7 +139,7 @@ def _p(a):
 
 def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inner=0,
               check_every=0, smoother=0, use_graph=1, line_tiles=(0, 0, 0, 0), fuse_cells=0,
-              streams=0, group_cells=0):
+              streams=0, group_cells=0, no_clamp=0):
     o = SgOpts()
     o.precision, o.rtol, o.max_iters = precision, rtol, max_iters
     o.inner_rtol, o.max_inner, o.check_every = inner_rtol, max_inner, check_every
@@ -143,7 +149,19 @@ def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inne
     o.fuse_cells = int(fuse_cells)
     o.streams = int(streams)
     o.group_cells = int(group_cells)
+    o.no_clamp = int(no_clamp)
     return o
+
+
+def _coeff_struct(keep, G=None, Dxx=None, Dxy=None, Dyy=None, inv_tau=None):
+    """SgCoeffs over contiguous copies (None -> NULL: that array is kept)."""
+    def k(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _p(a)
+    return SgCoeffs(k(G), k(Dxx), k(Dxy), k(Dyy), k(inv_tau))
 
 
 class _Inputs:
@@ -198,6 +216,12 @@ class Solver:
     def set_bc_values(self, side, vals):
         vals = np.ascontiguousarray(vals, dtype=np.float64)
         _check(lib().sg_set_bc_values(self.h, int(side), _p(vals)))
+
+    def set_coeffs(self, G=None, Dxx=None, Dxy=None, Dyy=None, inv_tau=None):
+        """New per-cell arrays, (nx, ny); None keeps an array's current values."""
+        keep = []
+        c = _coeff_struct(keep, G, Dxx, Dxy, Dyy, inv_tau)
+        _check(lib().sg_set_coeffs(self.h, C.byref(c)))
 
     def stats(self):
         s = SgStats()
@@ -278,6 +302,15 @@ class Batch:
         f = np.ascontiguousarray(f, dtype=np.float64)
         _check(lib().sg_batch_set_f(self.h, int(slot), _p(f), int(istep)))
 
+    def set_bc_values(self, slot, side, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        _check(lib().sg_batch_set_bc_values(self.h, int(slot), int(side), _p(vals)))
+
+    def set_coeffs(self, slot, G=None, Dxx=None, Dxy=None, Dyy=None, inv_tau=None):
+        keep = []
+        c = _coeff_struct(keep, G, Dxx, Dxy, Dyy, inv_tau)
+        _check(lib().sg_batch_set_coeffs(self.h, int(slot), C.byref(c)))
+
     def stats(self, slot):
         s = SgStats()
         _check(lib().sg_batch_stats(self.h, int(slot), C.byref(s)))
@@ -290,12 +323,18 @@ class Batch:
         return int(lib().sg_batch_bytes_in_use(self.h))
 
     def time_kernel(self, name):
-        _check(lib().sg_batch_time_kernel(self.h, KERNEL_IDS[name] if name else -1))
+        """Times every dispatch of one kernel family ("all": every family; None: off)."""
+        kid = -1 if not name else TIME_ALL if name == "all" else KERNEL_IDS[name]
+        _check(lib().sg_batch_time_kernel(self.h, kid))
 
-    def kernel_time(self):
-        """(total ms, launches, algorithmic bytes) of the kernel chosen by time_kernel."""
+    def kernel_time(self, name=None):
+        """(total ms, launches, algorithmic bytes) of the timed families, or of one."""
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()
-        _check(lib().sg_batch_kernel_time(self.h, C.byref(ms), C.byref(n), C.byref(b)))
+        if name is None:
+            _check(lib().sg_batch_kernel_time(self.h, C.byref(ms), C.byref(n), C.byref(b)))
+        else:
+            _check(lib().sg_batch_kernel_time_id(self.h, KERNEL_IDS[name], C.byref(ms), C.byref(n),
+                                                 C.byref(b)))
         return ms.value, n.value, b.value
 
     def close(self):

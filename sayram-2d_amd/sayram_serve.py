@@ -38,6 +38,7 @@ class Request:
     finished: float = math.nan
     result: np.ndarray | None = None
     stats: dict = field(default_factory=dict)
+    error: str | None = None   # why the request was rejected or stopped
 
     @property
     def latency(self) -> float:
@@ -79,11 +80,11 @@ class ServeLoop:
 
     def run(self, requests, realtime: bool = True, log=None):
         """Serves the requests; with realtime=False arrivals are released as
-        fast as the loop runs (a closed-loop backlog)."""
+        fast as the loop runs (a closed-loop backlog). Returns every request,
+        in completion order: a request that can never fit the HBM budget is
+        rejected on arrival, and one whose step the library rejects (the
+        solve did not converge) is stopped there; both carry `error`."""
         pending = sorted(requests, key=lambda r: r.arrival)
-        for r in pending:
-            if not self.fits_ever(r):
-                raise ValueError(f"request {r.rid} ({r.problem.nx}x{r.problem.ny}) exceeds the budget")
         queue = []                       # arrived, not admitted (FIFO)
         resident = {}                    # slot -> Request
         t0 = time.perf_counter()
@@ -93,8 +94,17 @@ class ServeLoop:
         while i < len(pending) or queue or resident:
             tnow = now()
             while i < len(pending) and (not realtime or pending[i].arrival <= tnow):
-                queue.append(pending[i])
+                r = pending[i]
                 i += 1
+                if self.fits_ever(r):
+                    queue.append(r)
+                else:
+                    r.error = (f"{r.problem.nx}x{r.problem.ny} needs more than the HBM budget "
+                               f"of {self.budget} bytes")
+                    r.finished = now()
+                    done.append(r)
+                    if log:
+                        log(f"[serve] request {r.rid} rejected: {r.error}")
             # admit in arrival order while the budget has room
             while queue:
                 slot = self.batch.try_add(queue[0].problem)
@@ -120,16 +130,20 @@ class ServeLoop:
             self.ticks += 1
             for s in list(resident):
                 st = self.batch.stats(s)
-                if st["steps"] >= resident[s].steps:
+                failed = st["status"] == sg.SG_ERR_NOCONV
+                if st["steps"] >= resident[s].steps or failed:
                     r = resident.pop(s)
                     r.result = self.batch.f(s)
                     r.stats = st
+                    if failed:
+                        r.error = (f"step {st['steps'] + 1} rejected: the solve did not converge "
+                                   f"(true residual {st['last_rel_residual']:.3e})")
                     self.batch.remove(s)
                     r.finished = now()
                     done.append(r)
                     if log:
                         log(f"[serve] request {r.rid} {r.problem.nx}x{r.problem.ny} x{r.steps} "
-                            f"done, latency {r.latency * 1e3:.1f} ms")
+                            f"{'failed' if failed else 'done'}, latency {r.latency * 1e3:.1f} ms")
         wall = now()
         return done, wall
 
@@ -138,11 +152,17 @@ class ServeLoop:
 
 
 def summarize(done, wall):
-    lat = np.array([r.latency for r in done])
-    cs = sum(r.cell_steps for r in done)
-    return {"requests": len(done), "wall_s": wall,
+    """Latency percentiles (arrival to result on the host, queueing included)
+    and throughput over the requests that completed; rejected or failed ones
+    are counted apart."""
+    ok = [r for r in done if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.array([math.nan])
+    cs = sum(r.cell_steps for r in ok)
+    return {"requests": len(done), "completed": len(ok), "failed": len(done) - len(ok),
+            "wall_s": wall,
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
             "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
             "latency_max_ms": float(lat.max() * 1e3),
             "throughput_cell_steps_per_s": cs / wall,
-            "min_f": float(min(r.result.min() for r in done))}
+            "throughput_req_per_s": len(ok) / wall,
+            "min_f": float(min(r.result.min() for r in ok)) if ok else math.nan}

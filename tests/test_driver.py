@@ -86,3 +86,49 @@ def test_driver_matches_oracle_and_writes_reference_layout(tmp_path):
     vals = np.array([float(x) for x in body.replace(",", " ").split()])
     assert vals.shape == (nx * ny,)
     assert rel_l2(vals.reshape(nx, ny), ref) <= 1e-10
+
+
+def test_driver_config_side_effects_without_gpu(tmp_path):
+    """Parameters' side effects and errors (source/Parameters.cc:7-23, 59-60):
+    no argument reads ./p.ini, the ini file is copied to
+    ./output/<run_id>/<run_id>.ini, and nsteps < nplots is an error instead of
+    0 steps with dt = T / 0. These run before the GPU is touched."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    _setup(tmp_path, 6, 5)
+    r = subprocess.run([DRIVER, "--case", "albert_young_lc"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    # ./p.ini was read (it names dID = none, whose table does not exist) and
+    # copied before the Equation was built
+    assert r.returncode == 2 and "D/none.h5" in r.stderr
+    assert (tmp_path / "output" / "drv" / "drv.ini").read_text() == (tmp_path / "p.ini").read_text()
+    bad = INI.format(run="few", nx=6, ny=5).replace("nsteps = 11", "nsteps = 3")
+    (tmp_path / "few.ini").write_text(bad)
+    r = subprocess.run([DRIVER, "few.ini", "prob.bin"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "less than nplots" in r.stderr
+    r = subprocess.run([DRIVER, "missing.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "cannot open missing.ini" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_copies_ini_and_reads_default_p_ini(tmp_path):
+    _setup(tmp_path, 12, 10)
+    (tmp_path / "p.ini").write_text(INI.format(run="dflt", nx=12, ny=10))
+    sc.write_problem(sc.seeded_case(12, 10, seed=77, scale=2.0, dt=0.0105 / 10),
+                     str(tmp_path / "prob.bin"))
+    r = subprocess.run([DRIVER, "p.ini", "prob.bin"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "output" / "dflt" / "dflt.ini").read_text() == (tmp_path / "p.ini").read_text()
+
+
+@pytest.mark.gpu
+def test_driver_reports_a_rejected_step(tmp_path):
+    """A step the library rejects reaches the driver as an exception from
+    GpuSolver::update (the reference throws std::runtime_error on errors)."""
+    _setup(tmp_path, 24, 20)
+    r = subprocess.run([DRIVER, "--max-iters", "1", "--rtol", "1e-16", "p.ini", "prob.bin"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "rejected" in r.stderr

@@ -122,10 +122,19 @@ def test_error_paths(gpu_lib):
     with pytest.raises(sg.SayramError) as e:
         sg.Solver(q)
     assert e.value.code == sg.SG_ERR_BC
+    # a step the solve cannot finish is rejected: the status reaches the
+    # caller, f and the step count stay as they were, and the next call with a
+    # reachable tolerance proceeds from there
     s = sg.Solver(p, sg.make_opts(max_iters=1, rtol=1e-16))
     rc = s.step_status(1)
-    assert rc == sg.SG_ERR_NOCONV and s.stats()["status"] == sg.SG_ERR_NOCONV
-    assert s.f().min() >= 0.0
+    st = s.stats()
+    assert rc == sg.SG_ERR_NOCONV and st["status"] == sg.SG_ERR_NOCONV
+    assert "rejected" in sg.lib().sg_last_error().decode()
+    assert st["steps"] == 0 and s.t() == 0.0 and st["last_nonfinite"] == 0
+    assert np.array_equal(s.f(), p.f0)
+    with pytest.raises(sg.SayramError) as e:
+        s.update(1)
+    assert e.value.code == sg.SG_ERR_NOCONV
     s.close()
     with pytest.raises(sg.SayramError) as e:
         sg.Batch(sg.make_opts(precision=3))
@@ -153,4 +162,27 @@ def test_batch_lifecycle_and_step_until(gpu_lib):
         ref.update(int(t))
         assert np.array_equal(b.f(sl), ref.f())
         ref.close()
+    b.close()
+
+
+def test_clamp_option(gpu_lib):
+    """f = max(x, 0) by default, f = x with no_clamp (the reference's copy-back,
+    source/Solver.cc:279-284): one step from the same state solves the same
+    system, so the two differ exactly by the clamp, and the clamp count is the
+    number of negative entries of x."""
+    sg = gpu_lib
+    p = sc.seeded_case(48, 40, seed=12, scale=4.0)
+    a = sg.Solver(p)
+    b = sg.Solver(p, sg.make_opts(no_clamp=1))
+    a.update(1)
+    b.update(1)
+    fb = b.f()
+    assert np.array_equal(a.f(), np.maximum(fb, 0.0))
+    sa, sb = a.stats(), b.stats()
+    assert sa["last_clamps"] == int(np.sum(fb < 0)) and sb["last_clamps"] == 0
+    assert sa["last_min_x"] == sb["last_min_x"] == fb.min()
+    assert sa["min_f"] == max(fb.min(), 0.0)
+    ref = orc.run(orc.OracleProblem(p), p.f0, 1)
+    assert rel_l2(fb, ref) <= 1e-10
+    a.close()
     b.close()
