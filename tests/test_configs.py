@@ -135,11 +135,15 @@ def test_serving_trace_from_the_serving_distribution(gpu_lib):
     reqs = sv.poisson_trace(24, 200.0, 32, 2048, 10, 1000, seed=2031)
     for r in reqs:
         r.steps = min(r.steps, 24)   # the trace's sizes; steps capped for test time
-    budget = 6 << 30
+    # a budget that holds a few of these requests at a time, so that most are
+    # admitted as earlier ones retire and join a running batch
+    budget = 640 << 20
     loop = sv.ServeLoop(hbm_budget=budget, precision=sg.SG_PC32)
     done, wall = loop.run(reqs, realtime=True)
     loop.close()
     assert len(done) == len(reqs) and all(r.error is None for r in done)
+    first = min(r.admitted for r in done)
+    assert sum(1 for r in done if r.admitted > first + 1e-9) > len(done) // 2
     assert max(max(r.problem.nx, r.problem.ny) for r in reqs) > 1024
     checked = 0
     for r in done:
