@@ -312,7 +312,7 @@ def main():
     ap.add_argument("--graph", choices=["on", "off"], default="on",
                     help="one captured hipGraph per step (per instance group)")
     ap.add_argument("--streams", type=int, choices=[0, 1, 2], default=0,
-                    help="HIP streams of the line sweeps' tile shapes (0: library default, 2)")
+                    help="HIP streams of the line sweeps' tile shapes (0: library default, 1)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch parameters on the workload first (sayram_tune) and "
                          "time the step with the best set")
@@ -418,7 +418,7 @@ def main():
                                   f"{args.big}^2 grid, dt = 1e-3 day",
                       "precision": args.precision,
                       "instances": len(probs), "cells": cells, "seed": args.seed},
-           "launch": {"graph": graph, "streams": args.streams or 2, "autotuned": tuned,
+           "launch": {"graph": graph, "streams": args.streams or 1, "autotuned": tuned,
                       "graph_launches_per_step": "one per instance group, plus pipelined chunks "
                                                  "when an instance needs more iterations than the "
                                                  "previous step's largest count"},

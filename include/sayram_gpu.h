@@ -103,14 +103,24 @@ typedef struct {
                            LDS); -1 disables. Like line_tiles it changes the
                            rounding of those line solves. */
   int streams;          /* 2: the tile shapes of one line sweep run on two HIP
-                           streams (the longest lines beside the rest); 1: one
-                           stream. 0 selects 2. Results are identical. */
+                           streams (the longest lines beside the rest), and the
+                           line factorisations beside the Galerkin chain; 1:
+                           one stream. 0 selects 1: with the large instances in
+                           a concurrent group of their own and a step captured
+                           as one graph, the second stream no longer pays
+                           (its fork and join edges slow the graph down).
+                           Results are identical. */
   int group_cells;      /* a batch steps its instances of at least group_cells
                            cells as a second group, on its own stream and host
                            thread, concurrently with the rest (their long line
                            solves are latency-bound, the others' work
                            bandwidth-bound). 0 selects 2^21; -1 keeps one group.
                            Results are identical. */
+  int head_iters;       /* FP64 and PC32 Krylov forms: BiCGSTAB iterations the
+                           step's first graph launch holds after the set-up
+                           (further iterations run in pipelined chunks of
+                           check_every). 0 (default): the previous step's
+                           largest count, at most 16; -1: none; n > 0: n. */
   int no_clamp;         /* 0 (default): the step ends with f = max(x, 0), so
                            min f >= 0 holds exactly; the reference copies the
                            solution unclamped (source/Solver.cc:279-284), whose

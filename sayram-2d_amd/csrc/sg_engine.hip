@@ -330,7 +330,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.check_every <= 0) o.check_every = 1;
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
-  if (o.streams == 0) o.streams = 2;
+  if (o.streams == 0) o.streams = 1;
   if (o.group_cells == 0) o.group_cells = 1 << 21;
   const int tiles[4] = {16, 160, 16, 768};
   for (int k = 0; k < 4; ++k)
@@ -1026,7 +1026,8 @@ int Batch::krylov(int maxit, bool xstate, int it0, bool ends) {
 // pipelined chunks of iterations, each with its own step-end pass, follow.
 template <typename KT, typename VT> int Batch::solve64() {
   const int maxit = opts.max_iters;
-  const int k0 = std::max(0, std::min({last_iters_max, maxit, kHeadIters}));
+  const int want = opts.head_iters < 0 ? 0 : opts.head_iters > 0 ? opts.head_iters : last_iters_max;
+  const int k0 = std::max(0, std::min({want, maxit, kHeadIters}));
   int rc = run_graph(graph_key(1, k0), [&] {
     setup_kernels(true);
     for (int k = 0; k < k0; ++k) iteration<KT, VT>(true);

@@ -41,7 +41,7 @@ class SgOpts(C.Structure):
                 ("inner_rtol", C.c_double), ("max_inner", C.c_int), ("check_every", C.c_int),
                 ("smoother", C.c_int), ("use_graph", C.c_int), ("line_tiles", C.c_int * 4),
                 ("fuse_cells", C.c_int), ("streams", C.c_int),
-                ("group_cells", C.c_int), ("no_clamp", C.c_int)]
+                ("group_cells", C.c_int), ("head_iters", C.c_int), ("no_clamp", C.c_int)]
 
 
 class SgStats(C.Structure):
@@ -139,7 +139,7 @@ def _p(a):
 
 def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inner=0,
               check_every=0, smoother=0, use_graph=1, line_tiles=(0, 0, 0, 0), fuse_cells=0,
-              streams=0, group_cells=0, no_clamp=0):
+              streams=0, group_cells=0, head_iters=0, no_clamp=0):
     o = SgOpts()
     o.precision, o.rtol, o.max_iters = precision, rtol, max_iters
     o.inner_rtol, o.max_inner, o.check_every = inner_rtol, max_inner, check_every
@@ -149,6 +149,7 @@ def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inne
     o.fuse_cells = int(fuse_cells)
     o.streams = int(streams)
     o.group_cells = int(group_cells)
+    o.head_iters = int(head_iters)
     o.no_clamp = int(no_clamp)
     return o
 
