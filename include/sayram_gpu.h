@@ -207,6 +207,15 @@ int sg_batch_step(sg_batch *b, int nsteps);
  * below target_steps[k] (indexed by slot; entries of free slots ignored). */
 int sg_batch_step_until(sg_batch *b, const int64_t *target_steps, int nslots, int nsteps);
 int sg_batch_get_f(sg_batch *b, int slot, double *out);
+/* sg_batch_get_f that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate device memory (code
+ * between graph captures, tight serving loops). It has no counterpart in the
+ * reference. The layout transpose is staged in a static device array of
+ * 4 Mi doubles (32 MiB) per GPU, held until the process ends; instances of
+ * more than 4 Mi cells are rejected with SG_ERR_ARG. Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from several host
+ * threads) race; calls from one thread are ordered. */
+int sg_batch_get_f_static(sg_batch *b, int slot, double *out);
 int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep);
 int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values);
 /* sg_set_coeffs for one instance of a batch. */

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -293,6 +294,7 @@ struct Batch {
   int read_counter(int which, int *out);
   int scratch(size_t n);
   int get_f(int slot, double *out);
+  int get_f_static(int slot, double *out, double *buf);
   int set_f(int slot, const double *f, int64_t istep);
   int debug_assemble(int slot, double *bands, double *R);
   // timing helpers
@@ -1259,6 +1261,63 @@ int Batch::debug_assemble(int slot, double *bands, double *R) {
 }  // namespace sg
 
 // ---------------------------------------------------------------------------
+// Static scratch of the no-allocation read-out (sg_batch_get_f_static): a
+// file-scope device array, one copy per GPU for the life of the process. Host
+// code never uses the array's name as an address: its device address on each
+// GPU comes from hipGetSymbolAddress, taken once per device with that device
+// current, when the first batch is created there.
+// ---------------------------------------------------------------------------
+namespace sg {
+
+constexpr size_t kStaticScratchElems = size_t(1) << 22;   // 4 Mi doubles, 32 MiB
+static __device__ __align__(256) double scratch[kStaticScratchElems];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (taken once per device).
+static int static_scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(SG_ERR_HIP, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return SG_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SG_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  return SG_OK;
+}
+
+static double *static_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return static_cast<double *>(g_scratch_ptr[dev]);
+}
+
+// sg_batch_get_f with the layout transpose staged in the static scratch:
+// no device allocation, whatever was read out before.
+int Batch::get_f_static(int slot, double *out, double *buf) {
+  if (slot < 0 || slot >= (int)slots.size() || !slots[slot].used)
+    return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  if (!out) return fail(SG_ERR_ARG, "out must not be null");
+  const Slot &s = slots[slot];
+  const size_t N = (size_t)s.nx * s.ny;
+  if (N > kStaticScratchElems)
+    return fail(SG_ERR_ARG, "instance of " + std::to_string(N) +
+                                " cells exceeds the static read-out scratch of " +
+                                std::to_string(kStaticScratchElems) + " cells");
+  launch_to_imajor(stream, s.dev.f, buf, s.nx, s.ny);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, buf, N * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SG_OK;
+}
+
+}  // namespace sg
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 using sg::Batch;
@@ -1374,6 +1433,8 @@ int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
   if (rc) return rc;
   b->opts = b->g[0].opts;   // with the defaults applied
   (void)hipGetDevice(&b->device);
+  // the device address of the static read-out scratch on this GPU
+  if ((rc = sg::static_scratch_setup(b->device))) return rc;
   *out = b.release();
   return SG_OK;
 }
@@ -1427,6 +1488,15 @@ int sg_batch_get_f(sg_batch *b, int slot, double *out) {
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
   return G->get_f(local, out);
+}
+int sg_batch_get_f_static(sg_batch *b, int slot, double *out) {
+  if (!b) return fail(SG_ERR_ARG, "null batch");
+  int local;
+  Batch *G = b->group_of(slot, &local);
+  if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
+  double *buf = sg::static_scratch(b->device);
+  if (!buf) return fail(SG_ERR_STATE, "static scratch not set up on this device");
+  return G->get_f_static(local, out, buf);
 }
 int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep) {
   if (!b) return fail(SG_ERR_ARG, "null batch");

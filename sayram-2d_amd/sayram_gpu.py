@@ -78,6 +78,7 @@ SYMBOLS = [
     ("sg_batch_step", C.c_int, [_V, C.c_int]),
     ("sg_batch_step_until", C.c_int, [_V, C.POINTER(C.c_int64), C.c_int, C.c_int]),
     ("sg_batch_get_f", C.c_int, [_V, C.c_int, _dp]),
+    ("sg_batch_get_f_static", C.c_int, [_V, C.c_int, _dp]),
     ("sg_batch_set_f", C.c_int, [_V, C.c_int, _dp, C.c_int64]),
     ("sg_batch_set_bc_values", C.c_int, [_V, C.c_int, C.c_int, _dp]),
     ("sg_batch_set_coeffs", C.c_int, [_V, C.c_int, C.POINTER(SgCoeffs)]),
@@ -297,6 +298,14 @@ class Batch:
         nx, ny = self.shapes[slot]
         out = np.empty((nx, ny))
         _check(lib().sg_batch_get_f(self.h, int(slot), _p(out)))
+        return out
+
+    def f_static(self, slot):
+        """f of one instance through the no-allocation read-out
+        (sg_batch_get_f_static)."""
+        nx, ny = self.shapes[slot]
+        out = np.empty((nx, ny))
+        _check(lib().sg_batch_get_f_static(self.h, int(slot), _p(out)))
         return out
 
     def set_f(self, slot, f, istep):
