@@ -96,6 +96,9 @@ typedef struct {
                            of at most [3] cells 32x32, longer columns 16x256. 0 selects the defaults 16, 160,
                            16, 768. They change the rounding of the line solves,
                            so results are reproducible for the same values. */
+  int coarse_tiles[4];  /* as line_tiles, for the line solves of the coarser
+                           levels (9-point operators); 0 selects the defaults
+                           16, 160, 16, 768. */
   int fuse_cells;       /* the V-cycle levels of an instance from its first level
                            of at most fuse_cells cells down run in one fused
                            launch per instance (post-smoothing schedules);

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -20,6 +21,15 @@
 namespace sg {
 
 static thread_local std::string g_err;
+
+// Stream capture and host-blocking HIP calls. A batch steps its two instance
+// groups on two host threads, and callers may drive several batches from
+// several threads, so one thread may capture a graph while another allocates,
+// copies synchronously or waits on a stream. Captures hold this lock
+// exclusively; host-blocking calls hold it shared, so no such call runs while
+// any thread is capturing.
+static std::shared_mutex g_capture_mu;
+using HostLock = std::shared_lock<std::shared_mutex>;
 
 static int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -236,6 +246,8 @@ struct Batch {
   int *d_cnt = nullptr;         // [0] active, [1] oactive
   int *h_cnt = nullptr;         // pinned
   int *h_pipe = nullptr;        // pinned, counters of the pipelined Krylov chunks
+  int *h_part = nullptr;        // pinned, participation flags being uploaded
+  int h_part_cap = 0;
   hipEvent_t ev_cnt[2] = {nullptr, nullptr};
   // Captured graphs by key: the per-step head (set-up, Krylov start, the first
   // iterations and a step-end pass) and chunks of iterations. They hold the map
@@ -312,6 +324,7 @@ Batch::~Batch() {
   if (h_cnt) (void)hipHostFree(h_cnt);
   if (h_pipe) (void)hipHostFree(h_pipe);
   if (h_scal) (void)hipHostFree(h_scal);
+  if (h_part) (void)hipHostFree(h_part);
   drop_graphs();
   for (auto ev : ev_cnt)
     if (ev) (void)hipEventDestroy(ev);
@@ -335,8 +348,10 @@ static void apply_defaults(sg_opts &o) {
   if (o.streams == 0) o.streams = 1;
   if (o.group_cells == 0) o.group_cells = 1 << 21;
   const int tiles[4] = {16, 160, 16, 768};
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 4; ++k) {
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
+    if (o.coarse_tiles[k] <= 0) o.coarse_tiles[k] = tiles[k];
+  }
 }
 
 int Batch::init(const sg_opts *o, size_t b) {
@@ -344,8 +359,12 @@ int Batch::init(const sg_opts *o, size_t b) {
   if (opts.precision != SG_FP64 && opts.precision != SG_MIXED && opts.precision != SG_PC32)
     return fail(SG_ERR_ARG, "opts.precision must be SG_FP64, SG_MIXED or SG_PC32");
   apply_defaults(opts);
-  if (opts.fuse_cells > kFuseCells)
-    return fail(SG_ERR_ARG, "opts.fuse_cells must be at most 1024 (the fused cycle keeps a level in LDS)");
+  if (opts.fuse_cells > (opts.precision == SG_FP64 ? kFuseCells : kFuseCells32))
+    return fail(SG_ERR_ARG, opts.precision == SG_FP64
+                                ? "opts.fuse_cells must be at most 1024 with FP64 V-cycles (the fused "
+                                  "cycle keeps a level in LDS)"
+                                : "opts.fuse_cells must be at most 4096 (the fused cycle keeps a level "
+                                  "in LDS)");
   if (opts.streams != 1 && opts.streams != 2) return fail(SG_ERR_ARG, "opts.streams must be 0, 1 or 2");
   budget = b;
   int ndev = 0;
@@ -614,6 +633,7 @@ int Batch::rebuild() {
     for (int l = 0; l < I.nlev; ++l) {
       const int lx = I.lev[l].nx, ly = I.lev[l].ny, Nl = lx * ly;
       const bool coarsest = (l == I.nlev - 1);
+      const int (&lim)[4] = l == 0 ? opts.line_tiles : opts.coarse_tiles;
       for (int k0 = 0; k0 < Nl; k0 += kBlock) {
         M.cmap[l].push_back({c, k0});
         if (!coarsest) M.pmap[l].push_back({c, k0});
@@ -625,17 +645,17 @@ int Batch::rebuild() {
         for (int k0 = 0; k0 < nq; k0 += kBlock) M.qmap[l].push_back({c, k0});
       if (coarsest) {
         LineGrp g{c, 0, 1, 1};
-        auto &set = ly == 1 ? M.gcx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)]
-                            : M.gcy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)];
-        auto &cyc = ly == 1 ? M.swx[l][tile_shape(opts.line_tiles, lx, 0, l, f64)]
-                            : M.swy[l][tile_shape(opts.line_tiles, ly, 1, l, f64)];
+        auto &set = ly == 1 ? M.gcx[l][tile_shape(lim, lx, 0, l, f64)]
+                            : M.gcy[l][tile_shape(lim, ly, 1, l, f64)];
+        auto &cyc = ly == 1 ? M.swx[l][tile_shape(lim, lx, 0, l, f64)]
+                            : M.swy[l][tile_shape(lim, ly, 1, l, f64)];
         set.push_back(g);
         if (l < lf) cyc.push_back(g);
       } else {
         // groups of tile_lines(shape) lines of one colour
         for (int d = 0; d < 2; ++d) {
           const int nl = d == 0 ? ly : lx;          // number of lines
-          const int sh = tile_shape(opts.line_tiles, d == 0 ? lx : ly, d, l, f64);
+          const int sh = tile_shape(lim, d == 0 ? lx : ly, d, l, f64);
           const int per = tile_lines(sh);
           for (int p = 0; p < 2; ++p)
             for (int f = p; f < nl; f += 2 * per) {
@@ -744,6 +764,7 @@ int Batch::rebuild() {
 
 int Batch::read_counter(int which, int *out) {
   HIPCHK(hipMemcpyAsync(h_cnt + which, d_cnt + which, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HostLock lk(g_capture_mu);
   HIPCHK(hipStreamSynchronize(stream));
   *out = h_cnt[which];
   return SG_OK;
@@ -771,6 +792,7 @@ void Batch::tend(int id, hipStream_t s) {
 
 int Batch::collect_time() {
   if (!ev_used) return SG_OK;
+  HostLock lk(g_capture_mu);
   HIPCHK(hipDeviceSynchronize());   // the side stream's events too
   for (size_t k = 0; k + 1 < ev_used; k += 2) {
     float ms = 0;
@@ -835,7 +857,7 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const bool deeper = l + 1 < m.maxlev && m.nrmap[l + 1] > 0;
   if (!deeper && m.nfmap) {
     tbegin(11);
-    launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm);
+    launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm, opts.fuse_cells);
     tend(11);
   }
   if (pre_x) sweep(0);
@@ -897,6 +919,7 @@ template <typename F> int Batch::run_graph(int key, F &&enqueue) {
   for (auto &g : graphs)
     if (g.key == key) exec = g.exec;
   if (!exec) {
+    std::unique_lock<std::shared_mutex> lk(g_capture_mu);
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     enqueue();
@@ -1014,7 +1037,10 @@ int Batch::krylov(int maxit, bool xstate, int it0, bool ends) {
   for (;;) {
     const bool more = it < maxit;
     if (more && (rc = chunk(slot ^ 1, it))) return rc;
-    HIPCHK(hipEventSynchronize(ev_cnt[slot]));
+    {
+      HostLock lk(g_capture_mu);
+      HIPCHK(hipEventSynchronize(ev_cnt[slot]));
+    }
     if (h_pipe[slot] == 0 || !more) break;
     slot ^= 1;
   }
@@ -1037,7 +1063,10 @@ template <typename KT, typename VT> int Batch::solve64() {
   });
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(h_pipe, d_cnt, sizeof(int), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipStreamSynchronize(stream));
+  {
+    HostLock lk(g_capture_mu);
+    HIPCHK(hipStreamSynchronize(stream));
+  }
   if (h_pipe[0] == 0 || k0 >= maxit) return SG_OK;
   return krylov<KT, VT>(maxit, true, k0, true);
 }
@@ -1112,7 +1141,17 @@ int Batch::step_once() {
   }
   if (!any) return SG_OK;
   if (part != part_host) {
-    HIPCHK(hipMemcpyAsync(d_part, part.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+    if (n > h_part_cap) {   // grows only between steps (pinned: the copy stays asynchronous)
+      HostLock lk(g_capture_mu);
+      HIPCHK(hipStreamSynchronize(stream));
+      if (h_part) (void)hipHostFree(h_part);
+      h_part = nullptr;
+      h_part_cap = 0;
+      HIPCHK(hipHostMalloc((void **)&h_part, sizeof(int) * n, hipHostMallocDefault));
+      h_part_cap = n;
+    }
+    std::memcpy(h_part, part.data(), n * sizeof(int));
+    HIPCHK(hipMemcpyAsync(d_part, h_part, n * sizeof(int), hipMemcpyHostToDevice, stream));
     part_host = part;
   }
   const bool mixed = opts.precision == SG_MIXED;
@@ -1139,7 +1178,10 @@ int Batch::step_once() {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(h_scal, d_scal, sizeof(KScal) * slots.size(), hipMemcpyDeviceToHost,
                         stream));
-  HIPCHK(hipStreamSynchronize(stream));
+  {
+    HostLock lk(g_capture_mu);
+    HIPCHK(hipStreamSynchronize(stream));
+  }
   if (time_mask) {
     if ((rc = collect_time())) return rc;
     count_bytes(part);
@@ -1176,7 +1218,10 @@ int Batch::step_once() {
 int Batch::step(int nsteps, const int64_t *targets, int ntargets) {
   int rc;
   if (nsteps < 0) return fail(SG_ERR_ARG, "nsteps must be >= 0");
-  if (dirty && (rc = rebuild())) return rc;
+  if (dirty) {
+    HostLock lk(g_capture_mu);
+    if ((rc = rebuild())) return rc;
+  }
   for (size_t k = 0; k < slots.size(); ++k)
     slots[k].target = (targets && (int)k < ntargets) ? targets[k] : -1;
   int worst = SG_OK;
@@ -1381,6 +1426,7 @@ struct sg_batch {
     // and wait on their own streams.
     for (int k = 0; k < 2; ++k)
       if (g[k].dirty) {
+        sg::HostLock lk(sg::g_capture_mu);
         const int rc = g[k].rebuild();
         if (rc) return rc;
       }
@@ -1424,6 +1470,7 @@ size_t sg_instance_bytes(int nx, int ny, int precision) {
 
 int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
   if (!out) return fail(SG_ERR_ARG, "out must not be null");
+  sg::HostLock lk(sg::g_capture_mu);
   *out = nullptr;
   std::unique_ptr<sg_batch> b(new sg_batch);
   if (opts) b->opts = *opts;
@@ -1442,6 +1489,7 @@ int sg_batch_create(const sg_opts *opts, size_t budget, sg_batch **out) {
 int sg_batch_add(sg_batch *b, const sg_grid *g, const sg_coeffs *c, const sg_bc *bc,
                  const double *f0, double dt, int *slot) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   if (!g) return fail(SG_ERR_ARG, "grid must not be null");
   const long long cells = (long long)g->nx * g->ny;
   const int k = (b->opts.group_cells > 0 && cells >= b->opts.group_cells) ? 1 : 0;
@@ -1463,6 +1511,7 @@ int sg_batch_add(sg_batch *b, const sg_grid *g, const sg_coeffs *c, const sg_bc 
 }
 int sg_batch_remove(sg_batch *b, int slot) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
@@ -1484,6 +1533,7 @@ int sg_batch_step_until(sg_batch *b, const int64_t *targets, int nslots, int nst
 }
 int sg_batch_get_f(sg_batch *b, int slot, double *out) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
@@ -1491,6 +1541,7 @@ int sg_batch_get_f(sg_batch *b, int slot, double *out) {
 }
 int sg_batch_get_f_static(sg_batch *b, int slot, double *out) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
@@ -1500,6 +1551,7 @@ int sg_batch_get_f_static(sg_batch *b, int slot, double *out) {
 }
 int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot " + std::to_string(slot));
@@ -1507,6 +1559,7 @@ int sg_batch_set_f(sg_batch *b, int slot, const double *f, int64_t istep) {
 }
 int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = b->group_of(slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance in slot");
@@ -1517,6 +1570,7 @@ int sg_batch_set_bc_values(sg_batch *b, int slot, int side, const double *values
 }
 int sg_batch_set_coeffs(sg_batch *b, int slot, const sg_coeffs *c) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
+  sg::HostLock lk(sg::g_capture_mu);
   int rc = sg::check_coeffs(c, false);
   if (rc) return rc;
   int local;
@@ -1534,7 +1588,11 @@ int sg_batch_stats(const sg_batch *b, int slot, sg_step_stats *out) {
 }
 int sg_batch_size(const sg_batch *b) { return b ? b->size() : 0; }
 size_t sg_batch_bytes_in_use(const sg_batch *b) { return b ? b->in_use() : 0; }
-void sg_batch_destroy(sg_batch *b) { delete b; }
+void sg_batch_destroy(sg_batch *b) {
+  if (!b) return;
+  sg::HostLock lk(sg::g_capture_mu);
+  delete b;
+}
 
 int sg_batch_time_kernel(sg_batch *b, int kernel_id) {
   if (!b) return fail(SG_ERR_ARG, "null batch");
@@ -1587,7 +1645,7 @@ int sg_create(const sg_grid *grid, const sg_coeffs *coeffs, const sg_bc *bc, con
   if (rc) return rc;
   int slot = -1;
   rc = sg_batch_add(b, grid, coeffs, bc, f0, dt, &slot);
-  if (rc) { delete b; return rc; }
+  if (rc) { sg_batch_destroy(b); return rc; }
   *out = new sg_solver{b, slot};
   return SG_OK;
 }
@@ -1625,11 +1683,12 @@ int sg_stats(const sg_solver *h, sg_step_stats *out) {
 }
 void sg_destroy(sg_solver *h) {
   if (!h) return;
-  delete h->bt;
+  sg_batch_destroy(h->bt);
   delete h;
 }
 int sg_debug_assemble(sg_solver *h, double *bands5, double *R) {
   if (!h || !bands5 || !R) return fail(SG_ERR_ARG, "null argument");
+  sg::HostLock lk(sg::g_capture_mu);
   int local;
   Batch *G = h->bt->group_of(h->slot, &local);
   if (!G) return fail(SG_ERR_ARG, "no instance");

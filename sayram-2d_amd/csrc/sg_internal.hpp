@@ -29,6 +29,7 @@ constexpr int kNumPartials = 5;
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
 constexpr int kLightBlocks = 2; // map entries per workgroup of the lightest vector kernels
+constexpr int kEnt = 4;         // map entries per workgroup of the multi-entry cell kernels
 // Tile shapes of the line solves: L lines x S steps staged through LDS by a
 // workgroup of NT threads, each thread owning 4 consecutive steps of a line in
 // the solve:
@@ -172,6 +173,8 @@ struct FuseMap { int inst; int lev; };    // an instance whose levels from lev d
 // first level with at most fuse_cells cells (the fused coarse cycle; default
 // kFuseCells).
 constexpr int kFuseCells = 1024;
+// the largest fused level an FP32 V-cycle can keep in LDS
+constexpr int kFuseCells32 = 4096;
 // Threads of the per-instance finish kernels (sums of the partials).
 constexpr int kFinThreads = 1024;
 constexpr int kFuseThreads = 1024;

@@ -1196,16 +1196,76 @@ __device__ __forceinline__ void prolong_point(const InstDev &I, int l, int xid, 
   x[K] = ZERO ? e : x[K] + e;
 }
 
+// A workgroup takes kEnt map entries. Each entry's chain of dependent reads
+// (map entry, instance, its status, its level's pointers) and then its loads
+// are issued before any entry is computed or stored, so a workgroup has
+// kEnt x 256 cells' loads in flight instead of waiting out one chain per 256
+// cells. The terms are those of prolong_point in its order (a zero weight of
+// a parent that point skips adds fma(0, 0, e) = e).
 template <typename T, bool ZERO, int ROWS>
 __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ insts,
                                                    const BlkMap *__restrict__ map, int l,
-                                                   int xid) {
-  const BlkMap m = map[blockIdx.x];
-  const InstDev &I = insts[m.inst];
-  if (rsc(I)->status != ST_ACTIVE) return;
-  const int idx = m.k0 + threadIdx.x;
-  if (idx >= prolong_count<ROWS>(I.lev[l])) return;
-  prolong_point<T, ZERO, ROWS>(I, l, xid, idx);
+                                                   int xid, int nmap) {
+  T a0[kEnt], a1[kEnt], a2[kEnt], a3[kEnt], c0[kEnt], c1[kEnt], c2[kEnt], c3[kEnt], xo[kEnt];
+  T *xs[kEnt];
+  int Ks[kEnt];
+  bool on[kEnt];
+#pragma unroll
+  for (int e = 0; e < kEnt; ++e) {
+    const int bi = kEnt * blockIdx.x + e;
+    on[e] = false;
+    a0[e] = a1[e] = a2[e] = a3[e] = c0[e] = c1[e] = c2[e] = c3[e] = xo[e] = T(0);
+    xs[e] = nullptr;
+    Ks[e] = 0;
+    if (bi >= nmap) continue;
+    const BlkMap m = map[bi];
+    const InstDev &I = insts[m.inst];
+    if (rsc(I)->status != ST_ACTIVE) continue;
+    const Lev &F = I.lev[l];
+    const Lev &Cl = I.lev[l + 1];
+    const int idx = m.k0 + threadIdx.x;
+    if (idx >= prolong_count<ROWS>(F)) continue;
+    const int hx = half_x(F.nx);
+    int K, i, j;
+    if (ROWS == 1) {            // rows 1, 3, ..: nx cells each
+      const int r = idx / F.nx, p = idx - r * F.nx;
+      j = 2 * r + 1; i = xcol(p, hx); K = j * F.nx + p;
+    } else if (ROWS == 2) {     // columns 1, 3, ..: the second half of each row
+      const int no = F.nx - hx;
+      j = idx / no;
+      const int p = hx + (idx - j * no);
+      i = xcol(p, hx); K = j * F.nx + p;
+    } else {
+      K = idx;
+      cell_ij(K, F.nx, hx, i, j);
+    }
+    const int I0 = i >> 1, J0 = j >> 1;
+    const int I1 = I0 + 1 < Cl.nx ? I0 + 1 : I0, J1 = J0 + 1 < Cl.ny ? J0 + 1 : J0;
+    const auto xc = gbl(cptr<T>(Cl.x));
+    const int hxc = half_x(Cl.nx);
+    const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
+    const int s0 = xslot(I0, hxc), s1 = xslot(I1, hxc);
+    const bool io = i & 1, jo = j & 1;
+    a0[e] = gbl(cptr<T>(F.pw[0]))[K];
+    c0[e] = xc[r0 + s0];
+    if (io) { a1[e] = gbl(cptr<T>(F.pw[1]))[K]; c1[e] = xc[r0 + s1]; }
+    if (jo) { a2[e] = gbl(cptr<T>(F.pw[2]))[K]; c2[e] = xc[r1 + s0]; }
+    if (io && jo) { a3[e] = gbl(cptr<T>(F.pw[3]))[K]; c3[e] = xc[r1 + s1]; }
+    T *x = l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x);
+    if (!ZERO) xo[e] = gbl(x)[K];
+    xs[e] = x;
+    Ks[e] = K;
+    on[e] = true;
+  }
+#pragma unroll
+  for (int e = 0; e < kEnt; ++e) {
+    if (!on[e]) continue;
+    T v = a0[e] * c0[e];
+    v += a1[e] * c1[e];
+    v += a2[e] * c2[e];
+    v += a3[e] * c3[e];
+    gbl(xs[e])[Ks[e]] = ZERO ? v : xo[e] + v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1223,29 +1283,28 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
 // runs the two Thomas recurrences out of LDS (lines are short here). The line
 // solves are the per-level ones rounded in another order.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int CAP>
 struct FuseLds {
-  T x[kFuseCells];     // x of the level being smoothed
-  T r[kFuseCells];     // line right-hand sides, then d of the forward pass
-  T lo[kFuseCells];    // coupling to the previous cell of the line (0 at t = 0)
-  T inv[kFuseCells];   // 1 / pivot
-  T up[kFuseCells];    // upper / pivot
+  T x[CAP];     // x of the level being smoothed
+  T r[CAP];     // line right-hand sides, then d of the forward pass
+  T lo[CAP];    // coupling to the previous cell of the line (0 at t = 0)
+  T inv[CAP];   // 1 / pivot
+  T up[CAP];    // upper / pivot
 };
 
-// FP32 levels are staged into LDS whole (b, the off-diagonal bands and the
-// line factors of both directions) once per level, so that the four sweeps
-// read no global memory; FP64 levels (twice the bytes) read them from global
-// memory in each sweep.
+// Levels of at most SCAP cells are staged into LDS whole (b, the off-diagonal
+// bands and the line factors of both directions) once per level, so that the
+// four sweeps read no global memory; larger levels (up to CAP cells) and FP64
+// levels read them from global memory in each sweep's parallel phase. The
+// variants keep a workgroup's LDS within 80 KB, two workgroups per CU:
+//   FP32: CAP 1024 with SCAP 1024 (72 KB), CAP 2048 with SCAP 512 (66 KB),
+//         CAP 4096 without staging (80 KB);  FP64: CAP 1024 (40 KB).
 enum FuseStageArr { FS_B, FS_W, FS_E, FS_S, FS_N, FS_SW, FS_SE, FS_NW, FS_NE, FS_I0, FS_U0, FS_I1,
                     FS_U1, FS_COUNT };
-template <typename T> struct FuseStage { T a[FS_COUNT][kFuseCells]; };
-struct FuseStageNone { char a[1][1]; };   // FP64: no staging
-template <typename T> struct FuseStageFor { using type = FuseStage<T>; };
-template <> struct FuseStageFor<double> { using type = FuseStageNone; };
-template <typename T> constexpr bool kFuseStaged = sizeof(T) == 4;
+template <typename T, int SCAP> struct FuseStage { T a[FS_COUNT][SCAP > 0 ? SCAP : 1]; };
 
-template <typename T, int NB>
-__device__ void fused_stage(const InstDev &I, int l, int bid, typename FuseStageFor<T>::type &G) {
+template <typename T, int NB, int SCAP>
+__device__ void fused_stage(const InstDev &I, int l, int bid, FuseStage<T, SCAP> &G) {
   const Lev &Lv = I.lev[l];
   const int nc = Lv.nx * Lv.ny;
   const auto b = gbl(l == 0 ? cptr<T>(vec(I, bid)) : cptr<T>(Lv.b));
@@ -1272,9 +1331,9 @@ __device__ void fused_stage(const InstDev &I, int l, int bid, typename FuseStage
 // are in G (LDS); otherwise they are read from global memory and the line
 // factors copied to S.lo, S.inv, S.up. Both read the same values and do the
 // same operations.
-template <typename T, int NB, int DIR, bool STG>
-__device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool zero, FuseLds<T> &S,
-                            typename FuseStageFor<T>::type &G) {
+template <typename T, int NB, int DIR, bool STG, int CAP, int SCAP>
+__device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool zero,
+                            FuseLds<T, CAP> &S, FuseStage<T, SCAP> &G) {
   const Lev &Lv = I.lev[l];
   const int nx = Lv.nx, ny = Lv.ny, hx = half_x(nx);
   const T *b = l == 0 ? cptr<T>(vec(I, bid)) : cptr<T>(Lv.b);
@@ -1363,8 +1422,8 @@ __device__ void fused_sweep(const InstDev &I, int l, int bid, int colour, bool z
 
 // x of level l: global -> LDS (load) or LDS -> global (store); ends with a
 // barrier.
-template <typename T>
-__device__ void fused_x(const InstDev &I, int l, int xid, FuseLds<T> &S, bool load) {
+template <typename T, int CAP>
+__device__ void fused_x(const InstDev &I, int l, int xid, FuseLds<T, CAP> &S, bool load) {
   const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(I.lev[l].x));
   const int nc = I.lev[l].nx * I.lev[l].ny;
   for (int K = threadIdx.x; K < nc; K += blockDim.x) {
@@ -1374,12 +1433,9 @@ __device__ void fused_x(const InstDev &I, int l, int xid, FuseLds<T> &S, bool lo
   __syncthreads();
 }
 
-template <typename T, int NB>
-__device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool post_x, bool post_y,
-                             FuseLds<T> &S, typename FuseStageFor<T>::type &G) {
-  constexpr bool STG = kFuseStaged<T>;
-  if constexpr (STG) fused_stage<T, NB>(I, l, bid, G);   // the barrier of fused_x covers it
-  fused_x<T>(I, l, xid, S, true);
+template <typename T, int NB, bool STG, int CAP, int SCAP>
+__device__ void fused_sweeps(const InstDev &I, int l, int bid, bool post_x, bool post_y,
+                             FuseLds<T, CAP> &S, FuseStage<T, SCAP> &G) {
   if (post_x) {
     fused_sweep<T, NB, 0, STG>(I, l, bid, 0, false, S, G);
     fused_sweep<T, NB, 0, STG>(I, l, bid, 1, false, S, G);
@@ -1388,15 +1444,31 @@ __device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool pos
     fused_sweep<T, NB, 1, STG>(I, l, bid, 0, false, S, G);
     fused_sweep<T, NB, 1, STG>(I, l, bid, 1, false, S, G);
   }
+}
+
+template <typename T, int NB, int CAP, int SCAP>
+__device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool post_x, bool post_y,
+                             FuseLds<T, CAP> &S, FuseStage<T, SCAP> &G) {
+  if constexpr (SCAP > 0) {
+    if (I.lev[l].nx * I.lev[l].ny <= SCAP) {   // uniform over the workgroup
+      fused_stage<T, NB>(I, l, bid, G);   // the barrier of fused_x covers it
+      fused_x<T>(I, l, xid, S, true);
+      fused_sweeps<T, NB, true>(I, l, bid, post_x, post_y, S, G);
+      fused_x<T>(I, l, xid, S, false);
+      return;
+    }
+  }
+  fused_x<T>(I, l, xid, S, true);
+  fused_sweeps<T, NB, false>(I, l, bid, post_x, post_y, S, G);
   fused_x<T>(I, l, xid, S, false);
 }
 
-template <typename T>
+template <typename T, int CAP, int SCAP>
 __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__restrict__ insts,
                                                                const FuseMap *__restrict__ map,
                                                                int xid, int bid, int smoother) {
-  __shared__ FuseLds<T> S;
-  __shared__ typename FuseStageFor<T>::type G;
+  __shared__ FuseLds<T, CAP> S;
+  __shared__ FuseStage<T, SCAP> G;
   const FuseMap fm = map[blockIdx.x];
   const InstDev &I = insts[fm.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
@@ -1439,13 +1511,20 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
 
 template <typename T>
 void launch_coarse_cycle(hipStream_t s, const InstDev *d, const FuseMap *m, int n, int xid, int bid,
-                         int smoother) {
-  if (n) k_coarse_cycle<T><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+                         int smoother, int cap) {
+  if (!n) return;
+  if constexpr (sizeof(T) == 8) {
+    k_coarse_cycle<T, 1024, 0><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+  } else {
+    if (cap <= 1024) k_coarse_cycle<T, 1024, 1024><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+    else if (cap <= 2048) k_coarse_cycle<T, 2048, 512><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+    else k_coarse_cycle<T, 4096, 0><<<n, kFuseThreads, 0, s>>>(d, m, xid, bid, smoother);
+  }
 }
 template void launch_coarse_cycle<double>(hipStream_t, const InstDev *, const FuseMap *, int, int,
-                                          int, int);
+                                          int, int, int);
 template void launch_coarse_cycle<float>(hipStream_t, const InstDev *, const FuseMap *, int, int,
-                                         int, int);
+                                         int, int, int);
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
@@ -2117,7 +2196,7 @@ template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid, int rows) {
   if (!nb) return;
-#define SG_PRO(Z, R) k_prolong<T, Z, R><<<nb, kBlock, 0, s>>>(d, m, l, xid)
+#define SG_PRO(Z, R) k_prolong<T, Z, R><<<(nb + kEnt - 1) / kEnt, kBlock, 0, s>>>(d, m, l, xid, nb)
   if (zero) { if (rows == 1) SG_PRO(true, 1); else if (rows == 2) SG_PRO(true, 2); else SG_PRO(true, 0); }
   else { if (rows == 1) SG_PRO(false, 1); else if (rows == 2) SG_PRO(false, 2); else SG_PRO(false, 0); }
 #undef SG_PRO

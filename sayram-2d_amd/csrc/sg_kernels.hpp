@@ -29,9 +29,11 @@ void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, in
                     int xid, int rows);
 // the fused coarse cycle of the instances in m (one workgroup each); post-
 // smoothing schedules only (smoother bits 4 and 8)
+// cap: the largest level a fused instance may have (fuse_cells); it selects
+// the LDS layout of the kernel
 template <typename T>
 void launch_coarse_cycle(hipStream_t s, const InstDev *d, const FuseMap *m, int n, int xid, int bid,
-                         int smoother);
+                         int smoother, int cap);
 // which: 0 p-update, 1 v = M ph, 2 s-update, 3 t = M sh, 4 x/r update.
 // pc: FP64 Krylov vectors with FP32 preconditioned vectors (T must be double).
 template <typename T>
