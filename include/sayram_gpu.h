@@ -87,6 +87,9 @@ typedef struct {
   int smoother;         /* bit0: pre x-lines, bit1: pre y-lines, bit2: post x,
                            bit3: post y; default 12 (post-smoothing, x-lines
                            then y-lines) */
+  int coarse_smoother;  /* smoother schedule of the levels below the finest:
+                           0 (default) as smoother; 4 post x-lines only, 8 post
+                           y-lines only, 12 both. Post-smoothing bits only. */
   int use_graph;        /* 0 (default) or 1: each chunk of check_every Krylov
                            iterations is captured once into a hipGraph and
                            replayed; -1: launch every kernel eagerly */
@@ -137,9 +140,9 @@ typedef struct {
 /* Acceptance of a step. The solve stops when ||R - M x||_2 <= rtol ||R||_2 (the
  * BiCGSTAB recurrence residual, or the FP64 residual of each refinement in
  * MIXED), or when that residual stops halving for 4 iterations (1 refinement)
- * while below 1e-9 relative: the FP64 rounding floor, about cond(M) eps, which
+ * while below 1e-10 relative: the FP64 rounding floor, about cond(M) eps, which
  * a direct solve reaches too. The step is then accepted only if the true
- * residual ||R - M x|| of the solution is within max(rtol, 1e-9) ||R||, every
+ * residual ||R - M x|| of the solution is within max(rtol, 1e-10) ||R||, every
  * x is finite and the solve did not break down or hit max_iters. A rejected
  * step leaves f and the step count unchanged and returns SG_ERR_NOCONV (the
  * reference's direct solve never produces an unconverged step). */

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -344,6 +345,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 1;
   if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
+  if (o.coarse_smoother == 0) o.coarse_smoother = o.smoother;
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 1;
   if (o.group_cells == 0) o.group_cells = 1 << 21;
@@ -366,6 +368,8 @@ int Batch::init(const sg_opts *o, size_t b) {
                                 : "opts.fuse_cells must be at most 4096 (the fused cycle keeps a level "
                                   "in LDS)");
   if (opts.streams != 1 && opts.streams != 2) return fail(SG_ERR_ARG, "opts.streams must be 0, 1 or 2");
+  if (opts.coarse_smoother != opts.smoother && (opts.coarse_smoother & ~12 || !(opts.coarse_smoother & 12)))
+    return fail(SG_ERR_ARG, "opts.coarse_smoother must be 4, 8 or 12 (post-smoothing)");
   budget = b;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -578,6 +582,9 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     }
   if (hipMemcpy(s.dev.f, fk.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "upload of f0 failed"));
+  // the scalars of a reused slot start from zero
+  if (hipMemsetAsync(d_scal + slot, 0, sizeof(KScal), stream) != hipSuccess)
+    return undo(fail(SG_ERR_HIP, "clearing the instance scalars failed"));
   s.used = true;
   in_use += bytes;
   dirty = true;
@@ -601,7 +608,7 @@ int Batch::remove(int slot) {
 // pre-smooths (the fused cycle implements post-smoothing) or when no level is
 // that small. It depends only on the instance and the options.
 int Batch::fuse_level(const InstDev &I) const {
-  if (opts.fuse_cells < 0 || (opts.smoother & 3)) return I.nlev;
+  if (opts.fuse_cells < 0 || ((opts.smoother | opts.coarse_smoother) & 3)) return I.nlev;
   for (int l = 0; l < I.nlev; ++l)
     if ((long long)I.lev[l].nx * I.lev[l].ny <= opts.fuse_cells) return l;
   return I.nlev;
@@ -617,7 +624,8 @@ int Batch::rebuild() {
   Maps M;
   order.clear();
   const bool f64 = opts.precision == SG_FP64;   // FP64 V-cycle
-  const int prow = prolong_rows(opts.smoother);
+  // the lines the first post-smoothing sweep of level l reads
+  auto prow_of = [&](int l) { return prolong_rows(l == 0 ? opts.smoother : opts.coarse_smoother); };
   std::vector<InstDev> table;
   int maxlev = 0;
   for (size_t k = 0; k < slots.size(); ++k) {
@@ -640,6 +648,7 @@ int Batch::rebuild() {
         if (l >= 1 && l <= lf) M.rmap[l].push_back({c, k0});        // restricted into from l - 1
       }
       // prolonged into from l + 1: only the lines the first post-sweep reads
+      const int prow = prow_of(l);
       const int nq = prow == 1 ? lx * (ly >> 1) : prow == 2 ? (lx - half_x(lx)) * ly : Nl;
       if (!coarsest && l < lf)
         for (int k0 = 0; k0 < nq; k0 += kBlock) M.qmap[l].push_back({c, k0});
@@ -834,7 +843,7 @@ template <typename F> void Batch::fan_out(const int (&cnt)[kNumTileShapes], F &&
 
 template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const DevMaps &m = dm;
-  const int sm = opts.smoother;
+  const int sm = l == 0 ? opts.smoother : opts.coarse_smoother;
   const bool pre_x = sm & 1, pre_y = sm & 2, post_x = sm & 4, post_y = sm & 8;
   const int lid = l == 0 ? 2 : 3;
   bool zero = true;
@@ -857,7 +866,8 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   const bool deeper = l + 1 < m.maxlev && m.nrmap[l + 1] > 0;
   if (!deeper && m.nfmap) {
     tbegin(11);
-    launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid, sm, opts.fuse_cells);
+    launch_coarse_cycle<T>(stream, d_insts, m.fmap, m.nfmap, xid, bid,
+                           opts.smoother | opts.coarse_smoother << 4, opts.fuse_cells);
     tend(11);
   }
   if (pre_x) sweep(0);
@@ -1082,8 +1092,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
   const double bv = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 4 + 8 : 64.0;
   const double bt = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 8 + 8 : 64.0;
   const double bvec = prec == SG_MIXED ? 60.0 : prec == SG_PC32 ? 24.0 + 28.0 + 52.0 : 120.0;
-  const int sm = opts.smoother;
-  const bool post_x = sm & 4;
+  auto post_x_of = [&](int l) { return ((l == 0 ? opts.smoother : opts.coarse_smoother) & 4) != 0; };
   for (size_t c = 0; c < part.size(); ++c) {
     if (!part[c]) continue;
     const Slot &s = slots[order[c]];
@@ -1109,16 +1118,18 @@ void Batch::count_bytes(const std::vector<int> &part) {
         t_bytes[3] += vc * Nl * 5.0 * T;
         continue;
       }
-      // a V-cycle sweeps every cell once per direction: 8 (5-point) or 12
-      // (9-point) values per swept cell
-      t_bytes[l == 0 ? 2 : 3] += vc * 2.0 * (l == 0 ? 8.0 : 12.0) * T * Nl;
+      // a V-cycle sweeps every cell once per direction it smooths: 8
+      // (5-point) or 12 (9-point) values per swept cell
+      const int sml = l == 0 ? opts.smoother : opts.coarse_smoother;
+      const double dirs = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0);
+      t_bytes[l == 0 ? 2 : 3] += vc * dirs * (l == 0 ? 8.0 : 12.0) * T * Nl;
       const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
       // restriction of b into level l + 1 (x = 0): r and the 4 weights per
       // fine cell, b out
       t_bytes[4] += vc * (5.0 * T * Nl + T * Nc);
       // prolongation into the lines the first sweep reads (half the cells):
       // their weights, x in and out, and the coarse x once
-      t_bytes[5] += vc * ((post_x ? 0.5 : 1.0) * Nl * 5.0 * T + T * Nc);
+      t_bytes[5] += vc * ((post_x_of(l) ? 0.5 : 1.0) * Nl * 5.0 * T + T * Nc);
     }
     t_bytes[6] += it * bv * N0;
     t_bytes[7] += it * bt * N0;

@@ -503,10 +503,15 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
     w2 = -(a.w * nW + a.n * wN + a.nw) / c;
     w3 = -(a.e * nE + a.n * eN + a.ne) / c;
   }
-  ptr<T>(L.pw[0])[K] = (T)w0;
-  ptr<T>(L.pw[1])[K] = (T)w1;
-  ptr<T>(L.pw[2])[K] = (T)w2;
-  ptr<T>(L.pw[3])[K] = (T)w3;
+  // Only the weights that are neither structurally zero nor exactly one are
+  // stored: an even-even point's single weight is 1 and its others are 0, and
+  // the parent at I0 + 1 of an even i (at J0 + 1 of an even j) has weight 0.
+  // Every reader takes those values as constants (rap_row, restrict_point,
+  // prolong_point, k_prolong), so the unstored entries are never read.
+  if (io || jo) ptr<T>(L.pw[0])[K] = (T)w0;
+  if (io) ptr<T>(L.pw[1])[K] = (T)w1;
+  if (jo) ptr<T>(L.pw[2])[K] = (T)w2;
+  if (io && jo) ptr<T>(L.pw[3])[K] = (T)w3;
 }
 
 // A_{l+1}(C, C') = sum_f sum_g P(f, C) A_l(f, g) P(g, C'); one work-item per
@@ -520,6 +525,7 @@ __global__ void __launch_bounds__(kBlock) k_pweights(const InstDev *__restrict__
 template <typename T, int NB>
 __device__ __forceinline__ void rap_row(const Lev &F, const T *const (&pw)[4], int Ic, int J,
                                         int fb, int hxf, double (&acc)[3][3]) {
+#pragma clang fp contract(off)
   constexpr int di_[9] = {0, -1, 1, 0, 0, -1, 1, -1, 1};
   constexpr int dj_[9] = {0, 0, 0, -1, 1, -1, -1, 1, 1};
   const int fx = F.nx, fy = F.ny;
@@ -530,9 +536,10 @@ __device__ __forceinline__ void rap_row(const Lev &F, const T *const (&pw)[4], i
     const int fi = 2 * Ic + fa;
     if (fi < 0 || fi >= fx) continue;
     const size_t kf = (size_t)fj * fx + xslot(fi, hxf);
-    // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
+    // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1));
+    // an even-even f (fa = fb = 0, slot 0) is injected with weight exactly 1
     const int slot = (fa < 0 ? 1 : 0) + 2 * (fb < 0 ? 1 : 0);
-    const double wf = (double)gbl(pw[slot])[kf];
+    const double wf = (fa == 0 && fb == 0) ? 1.0 : (double)gbl(pw[slot])[kf];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int ga = fa + di_[b], gb = fb + dj_[b];
@@ -550,7 +557,9 @@ __device__ __forceinline__ void rap_row(const Lev &F, const T *const (&pw)[4], i
         // gj (gi, gj have the parities of ga, gb): k_pweights writes exact
         // zeros there, and the sum skips them
         if (((q & 1) && !(ga & 1)) || ((q & 2) && !(gb & 1))) continue;
-        acc[aj][ai] += wa * (double)gbl(pw[q])[kg];
+        // an even-even g has the single weight 1 (q = 0 here)
+        const bool unit = !(ga & 1) && !(gb & 1);
+        acc[aj][ai] += wa * (unit ? 1.0 : (double)gbl(pw[q])[kg]);
       }
     }
   }
@@ -1102,7 +1111,8 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
       // slot of C among the parents of f: (I - (fi >> 1)) + 2 (J - (fj >> 1))
       const int slot = (da < 0 ? 1 : 0) + 2 * (db < 0 ? 1 : 0);
       const auto w = slot == 0 ? pw0 : slot == 1 ? pw1 : slot == 2 ? pw2 : pw3;
-      // the fine point at C itself is injected: k_pweights stores exactly 1
+      // the fine point at C itself is injected with weight exactly 1 (which
+      // k_pweights does not store)
       wv[db + 1][da + 1] = (da == 0 && db == 0) ? T(1) : T(w[kf]);
       rv[db + 1][da + 1] = r[kf];
       ok[db + 1][da + 1] = okj && oki;
@@ -1190,7 +1200,7 @@ __device__ __forceinline__ void prolong_point(const InstDev &I, int l, int xid, 
     e = a0 * c0;
     e += a2 * c2;
   } else {
-    e = w0[K] * xc[r0 + s0];
+    e = xc[r0 + s0];   // injection: the even-even weight is exactly 1
   }
   const auto x = gbl(l == 0 ? ptr<T>(vec(I, xid)) : ptr<T>(F.x));
   x[K] = ZERO ? e : x[K] + e;
@@ -1246,7 +1256,7 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
     const size_t r0 = (size_t)J0 * Cl.nx, r1 = (size_t)J1 * Cl.nx;
     const int s0 = xslot(I0, hxc), s1 = xslot(I1, hxc);
     const bool io = i & 1, jo = j & 1;
-    a0[e] = gbl(cptr<T>(F.pw[0]))[K];
+    a0[e] = (io || jo) ? gbl(cptr<T>(F.pw[0]))[K] : T(1);   // even-even: injected
     c0[e] = xc[r0 + s0];
     if (io) { a1[e] = gbl(cptr<T>(F.pw[1]))[K]; c1[e] = xc[r0 + s1]; }
     if (jo) { a2[e] = gbl(cptr<T>(F.pw[2]))[K]; c2[e] = xc[r1 + s0]; }
@@ -1463,6 +1473,8 @@ __device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool pos
   fused_x<T>(I, l, xid, S, false);
 }
 
+// smoother: the finest level's schedule in bits 0-3, the coarser levels' in
+// bits 4-7 (post-smoothing bits only)
 template <typename T, int CAP, int SCAP>
 __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__restrict__ insts,
                                                                const FuseMap *__restrict__ map,
@@ -1473,8 +1485,6 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   const InstDev &I = insts[fm.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
   const int lf = fm.lev, nl = I.nlev;
-  const bool post_x = smoother & 4, post_y = smoother & 8;
-  const int rows = prolong_rows(smoother);
   for (int m = lf; m + 1 < nl; ++m) {   // b_{m+1} = R b_m (x_m = 0)
     const int nc = I.lev[m + 1].nx * I.lev[m + 1].ny;
     for (int KC = threadIdx.x; KC < nc; KC += blockDim.x) restrict_point<T, true>(I, m, xid, bid, KC);
@@ -1493,6 +1503,10 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   }
   for (int m = nl - 2; m >= lf; --m) {  // x_m = P x_{m+1}, then post-smoothing
     const Lev &F = I.lev[m];
+    // schedule of level m: the finest level's (low 4 bits) or the coarser ones'
+    const int sml = m == 0 ? (smoother & 15) : (smoother >> 4);
+    const bool post_x = sml & 4, post_y = sml & 8;
+    const int rows = prolong_rows(sml);
     if (rows == 1) {
       const int np = prolong_count<1>(F);
       for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 1>(I, m, xid, k);
@@ -1956,11 +1970,12 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
         s.pcs = s.rnorm2 > 0.0 ? sqrt(s.rnorm2) : 1.0;
         s.beta = (double)(((T)s.rho / (T)s.rho_old) * ((T)s.alpha / (T)s.omega));
         // Converged; or stalled at the rounding floor: no halving of the
-        // residual for 4 iterations while below 1e-9 relative (or 100x the
-        // tolerance, for the loose inner solves of mixed precision).
+        // residual for 4 iterations while below 1e-10 relative (the per-step
+        // error budget; or 100x the tolerance, for the loose inner solves of
+        // mixed precision).
         bool floor = false;
         if (s.rnorm2 < 0.25 * s.best2) { s.best2 = s.rnorm2; s.best_it = s.iters; }
-        else floor = s.iters - s.best_it >= 4 && s.rnorm2 <= fmax(1e-18, 1e4 * s.tol2) * s.bnorm2;
+        else floor = s.iters - s.best_it >= 4 && s.rnorm2 <= fmax(1e-20, 1e4 * s.tol2) * s.bnorm2;
         if (s.rnorm2 <= s.tol2 * s.bnorm2 || floor) {
           s.status = ST_DONE;
           atomicSub(cnt, 1);
@@ -1981,10 +1996,10 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       s.scale = sqrt(t[0]);
       // Converged; or stalled at the FP64 rounding floor of R - M x, which is
       // about cond(M) eps and what a direct solve reaches too (the residual
-      // stopped halving while below 1e-9 relative).
+      // stopped halving while below 1e-10 relative).
       const bool conv = t[0] <= s.otol2 * s.obnorm2;
       const bool floor = !first && t[0] > 0.25 * s.oprev2 &&
-                         t[0] <= fmax(1e-18, 1e4 * s.otol2) * s.obnorm2;
+                         t[0] <= fmax(1e-20, 1e4 * s.otol2) * s.obnorm2;
       s.oprev2 = t[0];
       if (!isfinite(t[0]) || !isfinite(t[1])) {
         s.ostatus = ST_FAIL;
@@ -2041,9 +2056,9 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       s.minf = sm[0];
       // The step is accepted when the solve ended without failing, x is
       // finite and the true residual meets the tolerance, or the rounding
-      // floor the stall rule accepts (1e-9 relative) if that is larger.
+      // floor the stall rule accepts (1e-10 relative) if that is larger.
       const bool failed = I.prec == SG_MIXED ? ost == ST_FAIL : st == ST_FAIL;
-      const bool res_ok = t[0] <= fmax(s.otol2, 1e-18) * t[1];
+      const bool res_ok = t[0] <= fmax(s.otol2, 1e-20) * t[1];
       s.ostatus = (!failed && nf == 0.0 && res_ok) ? ST_DONE : ST_FAIL;
       s.phase = 1;
     }

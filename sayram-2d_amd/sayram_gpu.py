@@ -39,7 +39,8 @@ class SgBC(C.Structure):
 class SgOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("rtol", C.c_double), ("max_iters", C.c_int),
                 ("inner_rtol", C.c_double), ("max_inner", C.c_int), ("check_every", C.c_int),
-                ("smoother", C.c_int), ("use_graph", C.c_int), ("line_tiles", C.c_int * 4),
+                ("smoother", C.c_int), ("coarse_smoother", C.c_int), ("use_graph", C.c_int),
+                ("line_tiles", C.c_int * 4),
                 ("coarse_tiles", C.c_int * 4),
                 ("fuse_cells", C.c_int), ("streams", C.c_int),
                 ("group_cells", C.c_int), ("head_iters", C.c_int), ("no_clamp", C.c_int)]
@@ -141,11 +142,13 @@ def _p(a):
 
 def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inner=0,
               check_every=0, smoother=0, use_graph=1, line_tiles=(0, 0, 0, 0), fuse_cells=0,
-              streams=0, group_cells=0, head_iters=0, no_clamp=0, coarse_tiles=(0, 0, 0, 0)):
+              streams=0, group_cells=0, head_iters=0, no_clamp=0, coarse_tiles=(0, 0, 0, 0),
+              coarse_smoother=0):
     o = SgOpts()
     o.precision, o.rtol, o.max_iters = precision, rtol, max_iters
     o.inner_rtol, o.max_inner, o.check_every = inner_rtol, max_inner, check_every
     o.smoother, o.use_graph = smoother, use_graph
+    o.coarse_smoother = int(coarse_smoother)
     for k in range(4):
         o.line_tiles[k] = int(line_tiles[k])
         o.coarse_tiles[k] = int(coarse_tiles[k])

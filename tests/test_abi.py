@@ -62,8 +62,20 @@ def test_option_errors_come_before_any_device_call():
             sg.Batch(opts)
         assert e.value.code == sg.SG_ERR_ARG
     with pytest.raises(sg.SayramError) as e:
-        sg.Batch(sg.make_opts(fuse_cells=2048))
+        sg.Batch(sg.make_opts(fuse_cells=2048))          # FP64 V-cycles: at most 1024
     assert "fuse_cells" in str(e.value)
+    for opts in (sg.make_opts(precision=sg.SG_PC32, fuse_cells=4097),
+                 sg.make_opts(coarse_smoother=3), sg.make_opts(coarse_smoother=5)):
+        with pytest.raises(sg.SayramError) as e:
+            sg.Batch(opts)
+        assert e.value.code == sg.SG_ERR_ARG
+    if sg.lib().sg_device_count() == 0:
+        # valid options get as far as the missing GPU
+        for opts in (sg.make_opts(precision=sg.SG_PC32, fuse_cells=4096),
+                     sg.make_opts(coarse_smoother=4)):
+            with pytest.raises(sg.SayramError) as e:
+                sg.Batch(opts)
+            assert e.value.code == sg.SG_ERR_HIP
 
 
 def test_instance_bytes_accounting():

@@ -128,7 +128,10 @@ def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
     probs = [sc.seeded_case(nx, ny, seed=90 + k, scale=s) for k, (nx, ny, s) in
              enumerate([(24, 20, 2.0), (70, 45, 0.5), (33, 130, 5.0)])]
     out = {}
-    for fc in (0, -1, 300):
+    # FP32 V-cycles also keep levels of up to 2048 and 4096 cells in LDS
+    # (other LDS layouts of the fused kernel)
+    fcs = (0, -1, 300) if precision == 0 else (0, -1, 300, 2048, 4096)
+    for fc in fcs:
         b = gpu_lib.Batch(gpu_lib.make_opts(precision=precision, fuse_cells=fc))
         slots = [b.add(p) for p in probs]
         b.step(4)
@@ -136,19 +139,34 @@ def test_fused_coarse_cycle_matches_per_level_cycle(gpu_lib, precision):
         b.close()
     for k, p in enumerate(probs):
         ref = orc.run(orc.OracleProblem(p), p.f0, 4)
-        for fc in (0, 300):
-            assert rel_l2(out[fc][k], out[-1][k]) <= 1e-11
+        for fc in fcs:
+            if fc != -1:
+                assert rel_l2(out[fc][k], out[-1][k]) <= 1e-11
         for fc in out:
             assert rel_l2(out[fc][k], ref) <= TOL
+
+
+@pytest.mark.parametrize("csm", [4, 8])
+def test_coarse_smoother_schedules_stay_on_the_oracle(gpu_lib, csm):
+    """Coarse levels smoothed with x-lines only or y-lines only: a different
+    preconditioner, the same solution to the solve tolerance."""
+    probs = [sc.seeded_case(nx, ny, seed=95 + k, scale=s) for k, (nx, ny, s) in
+             enumerate([(70, 45, 0.5), (33, 130, 2.0), (120, 96, 1.0)])]
+    b = gpu_lib.Batch(gpu_lib.make_opts(precision=gpu_lib.SG_PC32, coarse_smoother=csm))
+    slots = [b.add(p) for p in probs]
+    b.step(3)
+    for p, sl in zip(probs, slots):
+        assert rel_l2(b.f(sl), orc.run(orc.OracleProblem(p), p.f0, 3)) <= TOL
+    b.close()
 
 
 @pytest.mark.parametrize("precision", ["pc32", "fp64"])
 @pytest.mark.parametrize("graph", [1, -1], ids=["graph", "eager"])
 def test_two_streams_equal_one_stream_bitwise(gpu_lib, graph, precision):
-    """The tile shapes of a sweep run on two streams by default, and the
+    """With streams = 2 the tile shapes of a sweep run on two streams, and the
     per-step line factorisations run on the second stream beside the chain of
     Galerkin products; they write disjoint data, so every result equals the
-    one-stream schedule bit for bit. Line lengths 12, 100, 300 and 900 give
+    one-stream schedule (the default) bit for bit. Line lengths 12, 100, 300 and 900 give
     sweeps with three tile shapes in both directions (FP64 V-cycles use other
     shapes than FP32 ones)."""
     prec = {"pc32": gpu_lib.SG_PC32, "fp64": gpu_lib.SG_FP64}[precision]
