@@ -118,6 +118,7 @@ def run_batched(probs, warmup, steps, precision, graph=True, opts=None, streams=
     if timing:
         for name in sg.KERNEL_IDS:
             fam[name] = b.kernel_time(name)
+        fam["_bracket_ms"] = b.event_overhead_ms()
     st = [b.stats(s) for s in slots]
     b.close()
     return dt, st, fam
@@ -386,13 +387,15 @@ def main():
         # one stream (same kernels, same launch sizes as the headline run)
         dte, _, fam = run_batched(probs, args.warmup, args.steps, prec, graph=False,
                                   opts=tuned_opts(-1), timing="all")
-        log(f"[bench] eager launches, one stream (kernel timing): {dte / args.steps * 1e3:.2f} ms/step")
+        bracket = fam.pop("_bracket_ms")
+        log(f"[bench] eager launches, one stream (kernel timing): {dte / args.steps * 1e3:.2f} ms/step; "
+            f"HIP-event interval around a one-workgroup kernel {bracket * 1e3:.1f} us")
         for name, (ms, nl, by) in sorted(fam.items(), key=lambda kv: -kv[1][0]):
             if nl:
                 log(f"[bench]   {name:13s} {ms / args.steps:7.2f} ms/step, {nl:6d} dispatches, "
                     f"mean {ms / nl * 1e3:7.1f} us" +
                     (f", {by / (ms * 1e-3) / 1e9:6.0f} GB/s algorithmic" if by > 0 else ""))
-        return dte, fam
+        return dte, fam, bracket
 
     if args.profile_only and not graph:
         kernel_timing_run()
@@ -429,7 +432,7 @@ def main():
                       "clamped_cells": int(sum(s["clamp_count"] for s in st)),
                       "rejected_steps": int(sum(s["status"] != 0 for s in st)),
                       "max_rel_residual": float(max(s["last_rel_residual"] for s in st))}}
-    dte, fam = kernel_timing_run()
+    dte, fam, bracket = kernel_timing_run()
     out["launch"]["eager_one_stream_ms_per_step"] = dte / args.steps * 1e3
     table = family_table(fam, args.steps)
     if table:
@@ -445,13 +448,16 @@ def main():
                                      "timed steps of an eager run on one stream and one instance "
                                      "group so that each interval is the kernel alone (the "
                                      "headline run overlaps tile shapes and instance groups); the "
-                                     "dominant family is the one with bytes and the most time"}
+                                     "dominant family is the one with bytes and the most time",
+                           "event_bracket_us": bracket * 1e3}
         tf = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
         if os.path.exists(tf):
             with open(tf) as fh:
                 t = json.load(fh)
             out["roofline"]["traffic"] = t["traffic_bytes_per_dispatch"]
             out["roofline"]["traffic_basis"] = t["basis"]
+            if "rocprof" in t:
+                out["roofline"]["rocprof"] = t["rocprof"]
         out["roofline_families"] = table
 
     if not args.skip_precisions:

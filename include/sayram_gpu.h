@@ -246,6 +246,9 @@ int sg_batch_kernel_time(sg_batch *b, double *total_ms, int64_t *launches, doubl
 /* Totals of one family (bytes 0 where no byte model applies). */
 int sg_batch_kernel_time_id(sg_batch *b, int kernel_id, double *total_ms, int64_t *launches,
                             double *bytes);
+/* Mean HIP-event interval around a one-workgroup kernel on the batch's stream:
+ * the bracket overhead included in every timed dispatch. */
+int sg_batch_event_overhead(sg_batch *b, double *ms);
 const char *sg_kernel_name(int kernel_id);
 int sg_kernel_count(void);
 int sg_solver_batch(sg_solver *h, sg_batch **out);

@@ -1642,6 +1642,35 @@ int sg_batch_kernel_time_id(sg_batch *b, int id, double *total_ms, int64_t *laun
   if (bytes) *bytes = b->g[0].t_bytes[id] + b->g[1].t_bytes[id];
   return SG_OK;
 }
+int sg_batch_event_overhead(sg_batch *b, double *ms) {
+  if (!b || !ms) return fail(SG_ERR_ARG, "null argument");
+  sg::HostLock lk(sg::g_capture_mu);
+  Batch &G = b->g[0];
+  constexpr int kPairs = 64;
+  hipEvent_t ev[2 * kPairs];
+  int made = 0;
+  for (; made < 2 * kPairs; ++made)
+    if (hipEventCreate(&ev[made]) != hipSuccess) break;
+  int rc = made == 2 * kPairs ? SG_OK : fail(SG_ERR_HIP, "hipEventCreate failed");
+  double tot = 0;
+  if (!rc) {
+    for (int k = 0; k < kPairs; ++k) {
+      (void)hipEventRecord(ev[2 * k], G.stream);
+      sg::launch_empty(G.stream, G.d_cnt);
+      (void)hipEventRecord(ev[2 * k + 1], G.stream);
+    }
+    if (hipStreamSynchronize(G.stream) != hipSuccess) rc = fail(SG_ERR_HIP, "synchronize failed");
+    for (int k = 0; !rc && k < kPairs; ++k) {
+      float t = 0;
+      if (hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]) != hipSuccess)
+        rc = fail(SG_ERR_HIP, "hipEventElapsedTime failed");
+      tot += t;
+    }
+  }
+  for (int k = 0; k < made; ++k) (void)hipEventDestroy(ev[k]);
+  if (!rc) *ms = tot / kPairs;
+  return rc;
+}
 const char *sg_kernel_name(int id) {
   return (id >= 0 && id < sg::kNumKernelIds) ? sg::kKernelNames[id] : "";
 }

@@ -2279,6 +2279,7 @@ void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
 void launch_commit(hipStream_t s, const InstDev *d, const BlkMap *m, int nb) {
   if (nb) k_commit<<<nb, kBlock, 0, s>>>(d, m);
 }
+void launch_empty(hipStream_t s, int *p) { k_zero_int<<<1, 64, 0, s>>>(p, 1); }
 void launch_count_nonfinite(hipStream_t s, const double *src, long long n, int *cnt) {
   k_zero_int<<<1, 64, 0, s>>>(cnt, 1);
   const long long nb = std::min<long long>(nblocks(n, kBlock), 2048);

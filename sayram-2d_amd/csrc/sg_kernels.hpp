@@ -51,6 +51,9 @@ void launch_ir_update(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 // and non-finite counts, min x); k_commit writes f of the accepted ones
 void launch_finalize(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
 void launch_commit(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
+// a one-workgroup kernel that zeroes *p (the dispatch whose HIP-event bracket
+// measures the timing overhead)
+void launch_empty(hipStream_t s, int *p);
 // *cnt = number of non-finite values in src[0 .. n)
 void launch_count_nonfinite(hipStream_t s, const double *src, long long n, int *cnt);
 void launch_to_korder(hipStream_t s, const double *src, double *dst, int nx, int ny);

@@ -94,6 +94,7 @@ SYMBOLS = [
                                        C.POINTER(C.c_double)]),
     ("sg_batch_kernel_time_id", C.c_int, [_V, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                           C.POINTER(C.c_double)]),
+    ("sg_batch_event_overhead", C.c_int, [_V, C.POINTER(C.c_double)]),
     ("sg_kernel_name", C.c_char_p, [C.c_int]),
     ("sg_kernel_count", C.c_int, []),
     ("sg_solver_batch", C.c_int, [_V, C.POINTER(_V)]),
@@ -351,6 +352,13 @@ class Batch:
             _check(lib().sg_batch_kernel_time_id(self.h, KERNEL_IDS[name], C.byref(ms), C.byref(n),
                                                  C.byref(b)))
         return ms.value, n.value, b.value
+
+    def event_overhead_ms(self):
+        """Mean HIP-event interval around a one-workgroup kernel on the batch's
+        stream (the bracket overhead of every timed dispatch)."""
+        ms = C.c_double()
+        _check(lib().sg_batch_event_overhead(self.h, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if self.h:

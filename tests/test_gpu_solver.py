@@ -2,6 +2,7 @@
 full-size properties (where the CPU oracle cannot follow), the error paths of
 the C ABI, resume, determinism and the batch lifecycle."""
 import glob
+import time
 import os
 
 import numpy as np
@@ -186,3 +187,40 @@ def test_clamp_option(gpu_lib):
     assert rel_l2(fb, ref) <= 1e-10
     a.close()
     b.close()
+
+
+def test_kernel_timing_changes_no_result(gpu_lib):
+    """HIP-event timing of every kernel family brackets each dispatch of an
+    eager run: results equal the untimed run bit for bit, every family that
+    ran reports dispatches, the timed intervals fit within the wall time, and
+    the bracket overhead is a positive interval."""
+    sg = gpu_lib
+    probs = [sc.seeded_case(nx, ny, seed=70 + k) for k, (nx, ny) in
+             enumerate([(40, 30), (130, 90), (64, 200)])]
+    out = []
+    for timed in (False, True):
+        b = sg.Batch(sg.make_opts(precision=sg.SG_PC32, use_graph=-1))
+        slots = [b.add(p) for p in probs]
+        b.step(1)
+        if timed:
+            b.time_kernel("all")
+        t0 = time.perf_counter()
+        b.step(2)
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        out.append([b.f(s) for s in slots])
+        if timed:
+            fam = {k: b.kernel_time(k) for k in sg.KERNEL_IDS}
+            for k in ("assemble", "rap", "line_l0", "spmv_v", "spmv_t", "vec_update",
+                      "finalize"):
+                assert fam[k][1] > 0 and fam[k][0] > 0, k
+            assert fam["assemble"][1] == 2           # one dispatch per step
+            assert fam["ir_residual"][1] == 0        # PC32 does no refinement
+            tot = b.kernel_time()
+            assert abs(tot[0] - sum(v[0] for v in fam.values())) <= 1e-9 * tot[0] + 1e-12
+            assert tot[1] == sum(v[1] for v in fam.values())
+            assert tot[0] <= wall_ms
+            ov = b.event_overhead_ms()
+            assert 0.0 < ov < 1.0
+        b.close()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
