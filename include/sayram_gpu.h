@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define SG_ABI_VERSION 3
+#define SG_ABI_VERSION 4
 
 typedef enum {
   SG_OK = 0,
@@ -135,13 +135,25 @@ typedef struct {
                            (sg_step_stats.last_clamps, clamp_count): a count
                            above 0 marks a step that departs from the
                            reference. 1: f = x, as the reference. */
+  int warm_start;       /* 0 (default): the solve of a step starts at f + df, df
+                           the increment of the instance's last accepted step
+                           (linear extrapolation in time; zero after creation
+                           and after sg_set_f / sg_batch_set_f), so a smooth
+                           trajectory needs fewer iterations; the solution is
+                           the same to the solve tolerance. -1: the solve
+                           starts at f, as a direct solve's result does not
+                           depend on a start, so a resumed run continues bit
+                           for bit. */
 } sg_opts;
 
 /* Acceptance of a step. The solve stops when ||R - M x||_2 <= rtol ||R||_2 (the
  * BiCGSTAB recurrence residual, or the FP64 residual of each refinement in
- * MIXED), or when that residual stops halving for 4 iterations (1 refinement)
- * while below 1e-10 relative: the FP64 rounding floor, about cond(M) eps, which
- * a direct solve reaches too. The step is then accepted only if the true
+ * MIXED), or when that residual has stalled at the FP64 rounding floor, about
+ * cond(M) eps, which a direct solve reaches too: no halving for 4 iterations
+ * while below 1e-12 relative, or for 8 while below 1e-10 (BiCGSTAB can
+ * plateau for several iterations above the floor before it converges on), or
+ * in MIXED one refinement that does not halve it below 1e-10. The step is
+ * then accepted only if the true
  * residual ||R - M x|| of the solution is within max(rtol, 1e-10) ||R||, every
  * x is finite and the solve did not break down or hit max_iters. A rejected
  * step leaves f and the step count unchanged and returns SG_ERR_NOCONV (the

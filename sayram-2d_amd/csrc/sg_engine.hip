@@ -84,6 +84,7 @@ static size_t carve(uintptr_t base, int nx, int ny, int prec, InstDev &I) {
   I.f = (double *)take(N * D); I.R = (double *)take(N * D); I.x = (double *)take(N * D);
   for (int b = 0; b < 5; ++b) I.band64[b] = (double *)take(N * D);
   I.r64 = mixed ? (double *)take(N * D) : nullptr;
+  I.df = (float *)take(N * sizeof(float));
   I.xe = (double *)take((nx + 1) * D); I.ye = (double *)take((ny + 1) * D);
   I.wx0 = (double *)take((nx + 1) * D); I.wx1 = (double *)take((nx + 1) * D);
   I.wy0 = (double *)take((ny + 1) * D); I.wy1 = (double *)take((ny + 1) * D);
@@ -546,6 +547,7 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
   carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, opts.precision, s.dev);
   s.dev.dt = dt;
   s.dev.noclamp = opts.no_clamp ? 1 : 0;
+  s.dev.warm = opts.warm_start >= 0 ? 1 : 0;
   s.dev.sc = d_scal + slot;
   s.dev.ticket = d_ticket + 2 * slot;
   s.dev.bcmask = 0;
@@ -582,6 +584,9 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     }
   if (hipMemcpy(s.dev.f, fk.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "upload of f0 failed"));
+  // no increment yet: the first solve starts at f0
+  if (hipMemsetAsync(s.dev.df, 0, N * sizeof(float), stream) != hipSuccess)
+    return undo(fail(SG_ERR_HIP, "clearing the step increment failed"));
   // the scalars of a reused slot start from zero
   if (hipMemsetAsync(d_scal + slot, 0, sizeof(KScal), stream) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "clearing the instance scalars failed"));
@@ -1100,7 +1105,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
     const InstDev &I = s.dev;
     const double N0 = (double)s.nx * s.ny, vc = k.vcyc, it = k.titers;
     t_bytes[0] += N0 * (48.0 + 56.0 + 20.0 * h32 + (prec == SG_MIXED ? 0.0 : 8.0) +
-                        (prec == SG_FP64 ? 24.0 : 0.0));
+                        (prec == SG_FP64 ? 24.0 : 0.0) + (I.warm ? 4.0 : 0.0));
     for (int l = 0; l < I.nlev; ++l) {
       const double Nl = (double)I.lev[l].nx * I.lev[l].ny;
       const double nb = l == 0 ? 5.0 : 9.0;
@@ -1135,7 +1140,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
     t_bytes[7] += it * bt * N0;
     t_bytes[8] += it * bvec * N0;
     if (prec == SG_MIXED) t_bytes[9] += (k.orefs + 1) * 64.0 * N0;
-    t_bytes[10] += N0 * (56.0 + 16.0);
+    t_bytes[10] += N0 * (56.0 + 16.0 + (I.warm ? 12.0 : 0.0));
   }
 }
 
@@ -1278,6 +1283,8 @@ int Batch::set_f(int slot, const double *f, int64_t istep) {
   if ((rc = scratch(N))) return rc;
   HIPCHK(hipMemcpyAsync(d_scratch, f, N * 8, hipMemcpyHostToDevice, stream));
   launch_to_korder(stream, d_scratch, s.dev.f, s.nx, s.ny);
+  // the state carries no previous step: the next solve starts at f
+  HIPCHK(hipMemsetAsync(s.dev.df, 0, N * sizeof(float), stream));
   HIPCHK(hipStreamSynchronize(stream));
   s.steps = istep;
   s.st.steps = istep;

@@ -143,7 +143,7 @@ struct InstDev {
   int nblk0;                  // level-0 workgroups of this instance
   int prec;                   // sg_precision of the instance
   int noclamp;                // 1: f = x at the end of a step (no clamp at 0)
-  int pad;
+  int warm;                   // 1: the solve starts at f + df, else at f
   double dt;
   // geometry (source/Mesh.cc:36-65) and per-vertex interpolation weights of
   // fill_vertex_from_cells (source/Solver.cc:326-381), computed once
@@ -155,6 +155,7 @@ struct InstDev {
   const double *G, *Dxx, *Dxy, *Dyy, *itau;
   // FP64 state
   double *f, *R, *x, *r64;
+  float *df;                  // f^n - f^(n-1) of the last accepted step (0: none yet)
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
   float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle inputs (s, p) and outputs,

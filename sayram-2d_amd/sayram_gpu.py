@@ -43,7 +43,8 @@ class SgOpts(C.Structure):
                 ("line_tiles", C.c_int * 4),
                 ("coarse_tiles", C.c_int * 4),
                 ("fuse_cells", C.c_int), ("streams", C.c_int),
-                ("group_cells", C.c_int), ("head_iters", C.c_int), ("no_clamp", C.c_int)]
+                ("group_cells", C.c_int), ("head_iters", C.c_int), ("no_clamp", C.c_int),
+                ("warm_start", C.c_int)]
 
 
 class SgStats(C.Structure):
@@ -143,7 +144,7 @@ def _p(a):
 
 def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inner=0,
               check_every=0, smoother=0, use_graph=1, line_tiles=(0, 0, 0, 0), fuse_cells=0,
-              streams=0, group_cells=0, head_iters=0, no_clamp=0, coarse_tiles=(0, 0, 0, 0),
+              streams=0, group_cells=0, head_iters=0, no_clamp=0, warm_start=0, coarse_tiles=(0, 0, 0, 0),
               coarse_smoother=0):
     o = SgOpts()
     o.precision, o.rtol, o.max_iters = precision, rtol, max_iters
@@ -158,6 +159,7 @@ def make_opts(precision=SG_FP64, rtol=0.0, max_iters=0, inner_rtol=0.0, max_inne
     o.group_cells = int(group_cells)
     o.head_iters = int(head_iters)
     o.no_clamp = int(no_clamp)
+    o.warm_start = int(warm_start)
     return o
 
 
@@ -219,6 +221,8 @@ class Solver:
 
     def set_f(self, f, istep):
         f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.shape != (self.nx, self.ny):
+            raise ValueError(f"f has shape {f.shape}, the instance {(self.nx, self.ny)}")
         _check(lib().sg_set_f(self.h, _p(f), int(istep)))
 
     def set_bc_values(self, side, vals):
@@ -316,6 +320,10 @@ class Batch:
 
     def set_f(self, slot, f, istep):
         f = np.ascontiguousarray(f, dtype=np.float64)
+        if slot not in self.shapes:
+            raise SayramError(SG_ERR_ARG, f"no instance in slot {slot}")
+        if f.shape != self.shapes[slot]:
+            raise ValueError(f"f has shape {f.shape}, the instance {self.shapes[slot]}")
         _check(lib().sg_batch_set_f(self.h, int(slot), _p(f), int(istep)))
 
     def set_bc_values(self, slot, side, vals):

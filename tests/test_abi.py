@@ -41,7 +41,7 @@ def test_python_front_end_binds_every_symbol():
 
 
 def test_abi_version():
-    assert sg.lib().sg_abi_version() == 3
+    assert sg.lib().sg_abi_version() == 4
 
 
 def test_no_gpu_fails_loudly():

@@ -83,24 +83,53 @@ def test_conservation_at_scale(gpu_lib):
         s.close()
 
 
-def test_resume_and_determinism(gpu_lib):
+@pytest.mark.parametrize("warm", [-1, 0], ids=["cold", "warm"])
+def test_resume_and_determinism(gpu_lib, warm):
+    """Resume = upload f and istep. With the solve started at f (warm_start
+    -1) the resumed run continues bit for bit; with the default warm start the
+    first resumed step has no increment to extrapolate, so it starts elsewhere
+    and agrees to the solve tolerance. Either way a run is bit-reproducible."""
     p = sc.seeded_case(70, 45, seed=7, scale=2.0)
-    a = gpu_lib.Solver(p)
+    opts = lambda: gpu_lib.make_opts(warm_start=warm)
+    a = gpu_lib.Solver(p, opts())
     a.update(6)
-    b = gpu_lib.Solver(p)
+    b = gpu_lib.Solver(p, opts())
     b.update(3)
     f3 = b.f()
     b.close()
-    c = gpu_lib.Solver(p)
-    c.set_f(f3, 3)          # resume = upload f and istep
+    c = gpu_lib.Solver(p, opts())
+    c.set_f(f3, 3)
     c.update(3)
-    assert np.array_equal(c.f(), a.f())
+    if warm < 0:
+        assert np.array_equal(c.f(), a.f())
+    else:
+        assert rel_l2(c.f(), a.f()) <= 1e-12
     assert abs(c.t() - 6 * p.dt) < 1e-15
-    d = gpu_lib.Solver(p)
+    d = gpu_lib.Solver(p, opts())
     d.update(6)
     assert np.array_equal(d.f(), a.f())   # bit-reproducible
     for s in (a, c, d):
         s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_warm_start_same_solution_fewer_iterations(gpu_lib, precision):
+    """Starting each solve at f + (last increment) changes the start, not the
+    system: both trajectories stay on the oracle, and on a smooth trajectory
+    the warm start needs no more iterations in total."""
+    sg = gpu_lib
+    p = sc.seeded_case(96, 80, seed=21, scale=1.0)
+    ref = orc.run(orc.OracleProblem(p), p.f0, 12)
+    its = {}
+    for warm in (-1, 0):
+        s = sg.Solver(p, sg.make_opts(precision=precision, warm_start=warm))
+        s.update(4)
+        t0 = s.stats()["total_iters"]
+        s.update(8)
+        its[warm] = s.stats()["total_iters"] - t0
+        assert rel_l2(s.f(), ref) <= 1e-10
+        s.close()
+    assert its[0] <= its[-1]
 
 
 def test_error_paths(gpu_lib):
@@ -224,3 +253,36 @@ def test_kernel_timing_changes_no_result(gpu_lib):
         b.close()
     for a, c in zip(out[0], out[1]):
         assert np.array_equal(a, c)
+
+
+def test_batch_resume_one_slot(gpu_lib):
+    """sg_batch_set_f resumes one instance of a batch from a saved state and
+    step count: it continues bit-identically to the uninterrupted run, and the
+    other instances are untouched. A state of the wrong shape is refused."""
+    sg = gpu_lib
+    probs = [sc.seeded_case(nx, ny, seed=80 + k, scale=s) for k, (nx, ny, s) in
+             enumerate([(50, 35, 1.0), (24, 60, 3.0), (41, 41, 0.5)])]
+    cold = sg.make_opts(warm_start=-1)     # solves start at f: resumes are bitwise
+    ref = sg.Batch(cold)
+    rs = [ref.add(p) for p in probs]
+    ref.step(2)
+    saved = ref.f(rs[1])
+    ref.step(3)
+    b = sg.Batch(cold)
+    bs = [b.add(p) for p in probs]
+    b.step(5)
+    b.set_f(bs[1], probs[1].f0, 0)      # restart slot 1 from f0 ...
+    b.step_until([5, 2, 5], 5)          # ... and run it back to step 2
+    assert b.stats(bs[1])["steps"] == 2
+    assert np.array_equal(b.f(bs[1]), saved)
+    b.set_f(bs[1], saved, 2)            # resume from the saved state
+    b.step_until([5, 5, 5], 5)
+    for r, s in zip(rs, bs):
+        assert np.array_equal(b.f(s), ref.f(r))
+        assert b.stats(s)["steps"] == 5
+    with pytest.raises(ValueError):
+        b.set_f(bs[0], np.zeros((3, 3)), 0)
+    with pytest.raises(sg.SayramError):
+        b.set_f(99, np.zeros((3, 3)), 0)
+    b.close()
+    ref.close()
