@@ -135,15 +135,16 @@ typedef struct {
                            (sg_step_stats.last_clamps, clamp_count): a count
                            above 0 marks a step that departs from the
                            reference. 1: f = x, as the reference. */
-  int warm_start;       /* 0 (default): the solve of a step starts at f + df, df
-                           the increment of the instance's last accepted step
-                           (linear extrapolation in time; zero after creation
-                           and after sg_set_f / sg_batch_set_f), so a smooth
-                           trajectory needs fewer iterations; the solution is
-                           the same to the solve tolerance. -1: the solve
-                           starts at f, as a direct solve's result does not
-                           depend on a start, so a resumed run continues bit
-                           for bit. */
+  int warm_start;       /* 0 or 1 (default): the solve of a step starts at
+                           f + df, df the increment of the instance's last
+                           accepted step (linear extrapolation in time; at f
+                           before the first accepted step since creation or
+                           since sg_set_f / sg_batch_set_f). A smooth
+                           trajectory then needs fewer iterations, and the
+                           solution is the same to the solve tolerance. -1:
+                           the solve starts at f, as a direct solve's result
+                           does not depend on a start, so a resumed run
+                           continues bit for bit. */
 } sg_opts;
 
 /* Acceptance of a step. The solve stops when ||R - M x||_2 <= rtol ||R||_2 (the

@@ -350,6 +350,7 @@ static void apply_defaults(sg_opts &o) {
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 1;
   if (o.group_cells == 0) o.group_cells = 1 << 21;
+  if (o.warm_start == 0) o.warm_start = 1;
   const int tiles[4] = {16, 160, 16, 768};
   for (int k = 0; k < 4; ++k) {
     if (o.line_tiles[k] <= 0) o.line_tiles[k] = tiles[k];
@@ -369,6 +370,8 @@ int Batch::init(const sg_opts *o, size_t b) {
                                 : "opts.fuse_cells must be at most 4096 (the fused cycle keeps a level "
                                   "in LDS)");
   if (opts.streams != 1 && opts.streams != 2) return fail(SG_ERR_ARG, "opts.streams must be 0, 1 or 2");
+  if (opts.warm_start < -1 || opts.warm_start > 1)
+    return fail(SG_ERR_ARG, "opts.warm_start must be -1, 0 or 1");
   if (opts.coarse_smoother != opts.smoother && (opts.coarse_smoother & ~12 || !(opts.coarse_smoother & 12)))
     return fail(SG_ERR_ARG, "opts.coarse_smoother must be 4, 8 or 12 (post-smoothing)");
   budget = b;
@@ -547,7 +550,7 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
   carve(reinterpret_cast<uintptr_t>(slab), g->nx, g->ny, opts.precision, s.dev);
   s.dev.dt = dt;
   s.dev.noclamp = opts.no_clamp ? 1 : 0;
-  s.dev.warm = opts.warm_start >= 0 ? 1 : 0;
+  s.dev.warm = opts.warm_start > 0 ? 1 : 0;
   s.dev.sc = d_scal + slot;
   s.dev.ticket = d_ticket + 2 * slot;
   s.dev.bcmask = 0;
@@ -584,9 +587,6 @@ int Batch::add(const sg_grid *g, const sg_coeffs *c, const sg_bc *bc, const doub
     }
   if (hipMemcpy(s.dev.f, fk.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "upload of f0 failed"));
-  // no increment yet: the first solve starts at f0
-  if (hipMemsetAsync(s.dev.df, 0, N * sizeof(float), stream) != hipSuccess)
-    return undo(fail(SG_ERR_HIP, "clearing the step increment failed"));
   // the scalars of a reused slot start from zero
   if (hipMemsetAsync(d_scal + slot, 0, sizeof(KScal), stream) != hipSuccess)
     return undo(fail(SG_ERR_HIP, "clearing the instance scalars failed"));
@@ -1105,7 +1105,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
     const InstDev &I = s.dev;
     const double N0 = (double)s.nx * s.ny, vc = k.vcyc, it = k.titers;
     t_bytes[0] += N0 * (48.0 + 56.0 + 20.0 * h32 + (prec == SG_MIXED ? 0.0 : 8.0) +
-                        (prec == SG_FP64 ? 24.0 : 0.0) + (I.warm ? 4.0 : 0.0));
+                        (prec == SG_FP64 ? 24.0 : 0.0) + 4.0 * std::min(I.warm, k.hist));
     for (int l = 0; l < I.nlev; ++l) {
       const double Nl = (double)I.lev[l].nx * I.lev[l].ny;
       const double nb = l == 0 ? 5.0 : 9.0;
@@ -1284,7 +1284,7 @@ int Batch::set_f(int slot, const double *f, int64_t istep) {
   HIPCHK(hipMemcpyAsync(d_scratch, f, N * 8, hipMemcpyHostToDevice, stream));
   launch_to_korder(stream, d_scratch, s.dev.f, s.nx, s.ny);
   // the state carries no previous step: the next solve starts at f
-  HIPCHK(hipMemsetAsync(s.dev.df, 0, N * sizeof(float), stream));
+  HIPCHK(hipMemsetAsync(&d_scal[slot].hist, 0, sizeof(int), stream));
   HIPCHK(hipStreamSynchronize(stream));
   s.steps = istep;
   s.st.steps = istep;

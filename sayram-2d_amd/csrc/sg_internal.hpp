@@ -125,7 +125,7 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
   int titers;           // Krylov iterations summed over the refinements of a step
   int vcyc;             // level-0 V-cycles this instance ran in the step
   int nonfin;           // finalize: cells whose x is not finite
-  int pad3;
+  int hist;             // 1: df holds the increment of an accepted step
 };
 
 struct Lev {            // one multigrid level of one instance
@@ -143,7 +143,7 @@ struct InstDev {
   int nblk0;                  // level-0 workgroups of this instance
   int prec;                   // sg_precision of the instance
   int noclamp;                // 1: f = x at the end of a step (no clamp at 0)
-  int warm;                   // 1: the solve starts at f + df, else at f
+  int warm;                   // 1: the solve starts at f + df, 0: at f
   double dt;
   // geometry (source/Mesh.cc:36-65) and per-vertex interpolation weights of
   // fill_vertex_from_cells (source/Solver.cc:326-381), computed once
@@ -155,7 +155,8 @@ struct InstDev {
   const double *G, *Dxx, *Dxy, *Dyy, *itau;
   // FP64 state
   double *f, *R, *x, *r64;
-  float *df;                  // f^n - f^(n-1) of the last accepted step (0: none yet)
+  float *df;                  // f^n - f^(n-1) of the last accepted step
+                              // (valid when KScal::hist is 1)
   double *band64[5];
   void *kv[kNumVec];          // Krylov vectors (double, or float in MIXED)
   float *pv[kNumVecAll - kNumVec];   // PC32: FP32 V-cycle inputs (s, p) and outputs,

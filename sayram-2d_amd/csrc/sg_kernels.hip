@@ -399,10 +399,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
       ptr<float>(L0.band[BS])[K] = (float)bs;
       ptr<float>(L0.band[BN])[K] = (float)bn;
     }
-    // the solve starts at f + df (the last step's increment: linear
-    // extrapolation in time), or at f
+    // the solve starts at f + df, df the increment of the last accepted step
+    // (linear extrapolation in time), or at f when there is none
     double x0K = fK, x0W = f3[1][0], x0E = f3[1][2], x0S = f3[0][1], x0N = f3[2][1];
-    if (I.warm) {
+    if (I.warm && rsc(I)->hist) {   // uniform over the workgroup
       const auto df = gbl(I.df);
       x0K += (double)df[K];
       x0W += (double)df[kw];
@@ -2080,6 +2080,8 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       const bool failed = I.prec == SG_MIXED ? ost == ST_FAIL : st == ST_FAIL;
       const bool res_ok = t[0] <= fmax(s.otol2, 1e-20) * t[1];
       s.ostatus = (!failed && nf == 0.0 && res_ok) ? ST_DONE : ST_FAIL;
+      // the commit that follows an accepted step records one more increment
+      if (s.ostatus == ST_DONE && I.warm) s.hist = 1;
       s.phase = 1;
     }
   }

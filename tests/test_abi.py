@@ -57,7 +57,8 @@ def test_no_gpu_fails_loudly():
 
 
 def test_option_errors_come_before_any_device_call():
-    for opts in (sg.make_opts(precision=3), sg.make_opts(fuse_cells=1025), sg.make_opts(streams=3)):
+    for opts in (sg.make_opts(precision=3), sg.make_opts(fuse_cells=1025), sg.make_opts(streams=3),
+                 sg.make_opts(warm_start=2), sg.make_opts(warm_start=-2)):
         with pytest.raises(sg.SayramError) as e:
             sg.Batch(opts)
         assert e.value.code == sg.SG_ERR_ARG
