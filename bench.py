@@ -296,8 +296,8 @@ def family_table(fam, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--lo", type=int, default=32)
     ap.add_argument("--hi", type=int, default=512)

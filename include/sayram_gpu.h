@@ -136,10 +136,12 @@ typedef struct {
                            above 0 marks a step that departs from the
                            reference. 1: f = x, as the reference. */
   int warm_start;       /* 0 or 1 (default): the solve of a step starts at
-                           f + df, df the increment of the instance's last
-                           accepted step (linear extrapolation in time; at f
-                           before the first accepted step since creation or
-                           since sg_set_f / sg_batch_set_f). A smooth
+                           f + theta df, df the increment of the instance's
+                           last accepted step and theta the ratio of its last
+                           two increments, (df_n, df_n-1) / (df_n-1, df_n-1)
+                           clamped to [0, 1] (an extrapolation in time; at f
+                           until two steps have been accepted since creation or
+                           since sg_set_f / sg_batch_set_f). A smoothly relaxing
                            trajectory then needs fewer iterations, and the
                            solution is the same to the solve tolerance. -1:
                            the solve starts at f, as a direct solve's result

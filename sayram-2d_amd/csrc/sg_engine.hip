@@ -1017,6 +1017,9 @@ void Batch::end_pass() {
   tbegin(10); launch_finalize(stream, d_insts, dm.cmap[0], nb0); tend(10);
   tbegin(12); launch_finish<double>(stream, FIN_FINALIZE, d_insts, n, d_cnt); tend(12);
   tbegin(10); launch_commit(stream, d_insts, dm.cmap[0], nb0); tend(10);
+  if (opts.warm_start > 0) {
+    tbegin(12); launch_finish<double>(stream, FIN_EXTRAP, d_insts, n, d_cnt); tend(12);
+  }
 }
 
 static int graph_key(int kind, int n) { return (kind << 16) | n; }
@@ -1105,7 +1108,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
     const InstDev &I = s.dev;
     const double N0 = (double)s.nx * s.ny, vc = k.vcyc, it = k.titers;
     t_bytes[0] += N0 * (48.0 + 56.0 + 20.0 * h32 + (prec == SG_MIXED ? 0.0 : 8.0) +
-                        (prec == SG_FP64 ? 24.0 : 0.0) + 4.0 * std::min(I.warm, k.hist));
+                        (prec == SG_FP64 ? 24.0 : 0.0) + (I.warm && k.theta > 0.0 ? 4.0 : 0.0));
     for (int l = 0; l < I.nlev; ++l) {
       const double Nl = (double)I.lev[l].nx * I.lev[l].ny;
       const double nb = l == 0 ? 5.0 : 9.0;
@@ -1140,7 +1143,7 @@ void Batch::count_bytes(const std::vector<int> &part) {
     t_bytes[7] += it * bt * N0;
     t_bytes[8] += it * bvec * N0;
     if (prec == SG_MIXED) t_bytes[9] += (k.orefs + 1) * 64.0 * N0;
-    t_bytes[10] += N0 * (56.0 + 16.0 + (I.warm ? 12.0 : 0.0));
+    t_bytes[10] += N0 * (56.0 + 16.0 + (I.warm ? 16.0 : 0.0));
   }
 }
 
@@ -1285,6 +1288,7 @@ int Batch::set_f(int slot, const double *f, int64_t istep) {
   launch_to_korder(stream, d_scratch, s.dev.f, s.nx, s.ny);
   // the state carries no previous step: the next solve starts at f
   HIPCHK(hipMemsetAsync(&d_scal[slot].hist, 0, sizeof(int), stream));
+  HIPCHK(hipMemsetAsync(&d_scal[slot].theta, 0, sizeof(double), stream));
   HIPCHK(hipStreamSynchronize(stream));
   s.steps = istep;
   s.st.steps = istep;

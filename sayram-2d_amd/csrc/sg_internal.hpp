@@ -117,6 +117,7 @@ struct KScal {          // per-instance Krylov scalars, FP64 storage
                          // written next), 2 finished in an earlier pass
   double true_rel;       // finalize: ||R - M x|| / ||R|| of the solution x
   double minf;           // finalize: min x (f before clamping)
+  double theta;          // warm start: the next solve starts at f + theta df
   int iters, status;     // Krylov iteration count and status of this solve
   int ostatus, orefs;    // mixed: outer refinement status and count; every
                          // precision: ST_FAIL after finalize rejects the step
@@ -143,7 +144,7 @@ struct InstDev {
   int nblk0;                  // level-0 workgroups of this instance
   int prec;                   // sg_precision of the instance
   int noclamp;                // 1: f = x at the end of a step (no clamp at 0)
-  int warm;                   // 1: the solve starts at f + df, 0: at f
+  int warm;                   // 1: the solve starts at f + theta df, 0: at f
   double dt;
   // geometry (source/Mesh.cc:36-65) and per-vertex interpolation weights of
   // fill_vertex_from_cells (source/Solver.cc:326-381), computed once

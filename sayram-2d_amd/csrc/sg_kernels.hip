@@ -399,16 +399,18 @@ __global__ void __launch_bounds__(kBlock) k_assemble(const InstDev *__restrict__
       ptr<float>(L0.band[BS])[K] = (float)bs;
       ptr<float>(L0.band[BN])[K] = (float)bn;
     }
-    // the solve starts at f + df, df the increment of the last accepted step
-    // (linear extrapolation in time), or at f when there is none
+    // the solve starts at f + theta df, df the increment of the last
+    // accepted step and theta the ratio of the last two increments (an
+    // extrapolation in time), or at f
     double x0K = fK, x0W = f3[1][0], x0E = f3[1][2], x0S = f3[0][1], x0N = f3[2][1];
-    if (I.warm && rsc(I)->hist) {   // uniform over the workgroup
+    const double th = rsc(I)->theta;   // uniform over the workgroup
+    if (I.warm && th > 0.0) {
       const auto df = gbl(I.df);
-      x0K += (double)df[K];
-      x0W += (double)df[kw];
-      x0E += (double)df[ke];
-      x0S += (double)df[ks];
-      x0N += (double)df[kn];
+      x0K += th * (double)df[K];
+      x0W += th * (double)df[kw];
+      x0E += th * (double)df[ke];
+      x0S += th * (double)df[ks];
+      x0N += th * (double)df[kn];
     }
     I.R[K] = R;
     I.x[K] = x0K;
@@ -1845,22 +1847,33 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const InstDev *__restrict__
 // (ostatus ST_FAIL: the solve broke down, hit its iteration cap, or left a
 // residual above the accepted floor, or x has non-finite cells) leaves f as it
 // was.
+// With the warm start it also keeps the step's increment df = f_new - f_old
+// and the partial sums (df_new, df_old), (df_old, df_old) from which F_EXTRAP
+// forms the next start's extrapolation factor.
 __global__ void __launch_bounds__(kBlock) k_commit(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
-  if (rsc(I)->phase != 1 || rsc(I)->ostatus != ST_DONE) return;
+  if (rsc(I)->phase != 1 || rsc(I)->ostatus != ST_DONE) return;   // uniform
   const int K = m.k0 + threadIdx.x;
-  if (K >= I.nx * I.ny) return;
-  const double xv = gbl(I.x)[K];
-  const double fn = (I.noclamp || xv > 0.0) ? xv : 0.0;
-  if (I.warm) gbl(I.df)[K] = (float)(fn - gbl(I.f)[K]);
-  gbl(I.f)[K] = fn;
+  double v[2] = {0.0, 0.0};
+  if (K < I.nx * I.ny) {
+    const double xv = gbl(I.x)[K];
+    const double fn = (I.noclamp || xv > 0.0) ? xv : 0.0;
+    if (I.warm) {
+      const double dold = (double)gbl(I.df)[K], dnew = fn - gbl(I.f)[K];
+      gbl(I.df)[K] = (float)dnew;
+      v[0] = dnew * dold;
+      v[1] = dold * dold;
+    }
+    gbl(I.f)[K] = fn;
+  }
+  if (I.warm) store_partials<double, 2>(I, m.k0 / kBlock, v);
 }
 
 // Finish of a reduction: one workgroup per instance sums the partial sums in a
 // fixed order and applies the BiCGSTAB / refinement scalar updates.
-enum Finish { F_INIT64 = 0, F_V, F_S, F_T, F_X, F_IR_RES, F_INNER_INIT, F_FINALIZE };
+enum Finish { F_INIT64 = 0, F_V, F_S, F_T, F_X, F_IR_RES, F_INNER_INIT, F_FINALIZE, F_EXTRAP };
 
 template <typename A, int NV>
 __device__ __forceinline__ void sum_partials(const InstDev &I, A (&tot)[NV]) {
@@ -1903,6 +1916,8 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
   const int st = s.status, ost = s.ostatus;
   if (MODE == F_IR_RES || MODE == F_INNER_INIT) {
     if (ost != ST_ACTIVE) return;
+  } else if (MODE == F_EXTRAP) {
+    if (s.phase != 1 || ost != ST_DONE) return;   // committed in this pass
   } else if (MODE == F_FINALIZE) {
     if (s.phase == 1) {   // committed by the previous pass
       if (threadIdx.x == 0) s.phase = 2;
@@ -2047,6 +2062,21 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       s.best2 = t[0]; s.best_it = 0;
       s.status = ST_ACTIVE;
     }
+  } else if (MODE == F_EXTRAP) {
+    // The next solve starts at f + theta df: theta = (df_new, df_old) /
+    // (df_old, df_old), clamped to [0, 1], the decay ratio of the increments,
+    // once two increments are known (a start from f before). A trajectory
+    // relaxing smoothly gets theta near 1; the fast transients of the first
+    // steps, whose increments shrink or turn from step to step, get a small
+    // theta instead of an overshooting start.
+    double t[2];
+    sum_partials<double, 2>(I, t);
+    if (threadIdx.x == 0) {
+      const double r = t[1] > 0.0 ? t[0] / t[1] : 0.0;
+      s.theta = s.hist ? fmin(fmax(r, 0.0), 1.0) : 0.0;
+      if (!isfinite(s.theta)) s.theta = 0.0;
+      s.hist = 1;
+    }
   } else {   // F_FINALIZE
     double t[4];
     sum_partials<double, 4>(I, t);
@@ -2081,7 +2111,6 @@ __global__ void __launch_bounds__(kFinThreads) k_finish(const InstDev *__restric
       const bool res_ok = t[0] <= fmax(s.otol2, 1e-20) * t[1];
       s.ostatus = (!failed && nf == 0.0 && res_ok) ? ST_DONE : ST_FAIL;
       // the commit that follows an accepted step records one more increment
-      if (s.ostatus == ST_DONE && I.warm) s.hist = 1;
       s.phase = 1;
     }
   }
@@ -2280,7 +2309,8 @@ void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt) {
     case F_X: k_finish<T, F_X><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
     case F_IR_RES: k_finish<T, F_IR_RES><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
     case F_INNER_INIT: k_finish<T, F_INNER_INIT><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
-    default: k_finish<T, F_FINALIZE><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    case F_FINALIZE: k_finish<T, F_FINALIZE><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
+    default: k_finish<T, F_EXTRAP><<<n, kFinThreads, 0, s>>>(d, n, cnt); break;
   }
 }
 template void launch_finish<double>(hipStream_t, int, const InstDev *, int, int *);

@@ -40,7 +40,8 @@ template <typename T>
 void launch_bicg(hipStream_t s, int which, const InstDev *d, const BlkMap *m, int nb, bool xstate,
                  bool pc);
 // reduction finish modes (one workgroup per instance)
-enum { FIN_INIT64 = 0, FIN_V, FIN_S, FIN_T, FIN_X, FIN_IR_RES, FIN_INNER_INIT, FIN_FINALIZE };
+enum { FIN_INIT64 = 0, FIN_V, FIN_S, FIN_T, FIN_X, FIN_IR_RES, FIN_INNER_INIT, FIN_FINALIZE,
+       FIN_EXTRAP };
 template <typename T>
 void launch_finish(hipStream_t s, int mode, const InstDev *d, int n, int *cnt);
 void launch_ir_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb);
