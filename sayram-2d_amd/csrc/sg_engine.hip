@@ -193,6 +193,29 @@ template <typename G> static void xcd_chunks(std::vector<G> &v) {
   v.swap(out);
 }
 
+// The cell-parallel maps (256 cells of one instance per entry, instances one
+// after another, each in row order) are reordered so that XCD x takes one
+// contiguous eighth of the map: workgroup w (on XCD w % 8) takes unit
+// start(x) + w / 8, a unit being the `unit` consecutive entries one workgroup
+// of the kernel takes. The blocks of neighbouring rows of an instance then
+// run on one XCD and read their halo rows (stencils, restriction, Galerkin
+// products) from its L2 instead of fetching them again. A last partial unit
+// keeps its place at the end. No result depends on the order: every block
+// writes its own cells and its own partial-sum slot.
+static void xcd_blocks(std::vector<BlkMap> &v, int unit) {
+  constexpr size_t kXcd = 8;
+  const size_t nu = v.size() / unit;   // full units
+  if (nu < 2 * kXcd) return;
+  const size_t base = nu / kXcd, rem = nu % kXcd;
+  std::vector<BlkMap> out(v);
+  for (size_t w = 0; w < nu; ++w) {
+    const size_t x = w % kXcd, j = w / kXcd;
+    const size_t u = x * base + std::min(x, rem) + j;
+    for (int e = 0; e < unit; ++e) out[w * unit + e] = v[u * unit + e];
+  }
+  v.swap(out);
+}
+
 struct DevMaps {   // offsets into one device buffer
   const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
   int ncmap[kMaxLev], npmap[kMaxLev];
@@ -698,6 +721,12 @@ int Batch::rebuild() {
             if (d == 1) xcd_chunks(*w);
           }
         }
+  for (int l = 0; l < kMaxLev; ++l) {
+    xcd_blocks(M.cmap[l], 1);
+    xcd_blocks(M.pmap[l], 1);
+    xcd_blocks(M.rmap[l], 1);
+    xcd_blocks(M.qmap[l], kEnt);   // k_prolong takes kEnt entries per workgroup
+  }
   // fused cycles with the most levels first: they are the critical path
   std::stable_sort(M.fmap.begin(), M.fmap.end(), [&](const FuseMap &a, const FuseMap &b) {
     return table[a.inst].nlev - a.lev > table[b.inst].nlev - b.lev;
@@ -1100,7 +1129,9 @@ void Batch::count_bytes(const std::vector<int> &part) {
   const double bv = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 4 + 8 : 64.0;
   const double bt = prec == SG_MIXED ? 16.0 + 4 * 3 : prec == SG_PC32 ? 40.0 + 4 + 8 + 8 : 64.0;
   const double bvec = prec == SG_MIXED ? 60.0 : prec == SG_PC32 ? 24.0 + 28.0 + 52.0 : 120.0;
-  auto post_x_of = [&](int l) { return ((l == 0 ? opts.smoother : opts.coarse_smoother) & 4) != 0; };
+  auto half_prolong = [&](int l) {
+    return prolong_rows(l == 0 ? opts.smoother : opts.coarse_smoother) != 0;
+  };
   for (size_t c = 0; c < part.size(); ++c) {
     if (!part[c]) continue;
     const Slot &s = slots[order[c]];
@@ -1116,9 +1147,11 @@ void Batch::count_bytes(const std::vector<int> &part) {
       t_bytes[1] += Nl * (5.0 * 8.0 + 4.0 * T);
       if (l + 1 < I.nlev) {
         const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
-        // prolongation weights (bands in, 4 weights out) and the Galerkin
-        // product (bands and weights in, 9 FP64 bands and their copies out)
-        t_bytes[1] += Nl * (nb * 8.0 + 4.0 * T) + Nl * (nb * 8.0 + 4.0 * T) +
+        // prolongation weights (bands in, the stored weights out: 0, 2, 2
+        // and 4 for the even-even, odd-even, even-odd and odd-odd points, 2
+        // per point on average) and the Galerkin product (bands and stored
+        // weights in, 9 FP64 bands and their copies out)
+        t_bytes[1] += Nl * (nb * 8.0 + 2.0 * T) + Nl * (nb * 8.0 + 2.0 * T) +
                       Nc * (72.0 + 36.0 * h32);
       }
       if (l >= s.fuse_lev) continue;   // the fused coarse cycle
@@ -1132,12 +1165,14 @@ void Batch::count_bytes(const std::vector<int> &part) {
       const double dirs = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0);
       t_bytes[l == 0 ? 2 : 3] += vc * dirs * (l == 0 ? 8.0 : 12.0) * T * Nl;
       const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
-      // restriction of b into level l + 1 (x = 0): r and the 4 weights per
-      // fine cell, b out
-      t_bytes[4] += vc * (5.0 * T * Nl + T * Nc);
-      // prolongation into the lines the first sweep reads (half the cells):
-      // their weights, x in and out, and the coarse x once
-      t_bytes[5] += vc * ((post_x_of(l) ? 0.5 : 1.0) * Nl * 5.0 * T + T * Nc);
+      // restriction of b into level l + 1 (x = 0): r and the stored weights
+      // (2 per fine cell on average) in, b out
+      t_bytes[4] += vc * (3.0 * T * Nl + T * Nc);
+      // prolongation into the lines the first sweep reads (half the cells:
+      // the odd rows or columns, 3 stored weights per cell on average), x in
+      // and out, and the coarse x once; into every cell (2 weights on
+      // average) when the first sweep is not a post-smoothing line sweep
+      t_bytes[5] += vc * ((half_prolong(l) ? 0.5 * 5.0 : 4.0) * Nl * T + T * Nc);
     }
     t_bytes[6] += it * bv * N0;
     t_bytes[7] += it * bt * N0;
