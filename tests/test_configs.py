@@ -47,7 +47,7 @@ def test_batched_workload_every_instance_every_step(gpu_lib):
     for name, pv in precs.items():
         b = sg.Batch(sg.make_opts(precision=pv))
         batches[name] = (b, [b.add(p) for p in probs])
-        print(f"[c4] {name}: {len(probs)} instances resident, {b.bytes_in_use() / 2**30:.1f} GiB",
+        print(f"[batch] {name}: {len(probs)} instances resident, {b.bytes_in_use() / 2**30:.1f} GiB",
               flush=True)
     small = sorted(range(len(probs)), key=lambda k: probs[k].ncells)[:12]
     fo = {k: probs[k].f0.copy() for k in small}
@@ -78,7 +78,7 @@ def test_batched_workload_every_instance_every_step(gpu_lib):
                     e = rel_l2(f[name], fo[k])
                     worst["oracle"] = max(worst["oracle"], e)
                     assert e <= TOL, f"{name} vs oracle, instance {k}, step {step + 1}: {e:.3e}"
-        print(f"[c4] step {step + 1}: worst true residual {worst['res']:.2e}, pc32 vs fp64 "
+        print(f"[batch] step {step + 1}: worst true residual {worst['res']:.2e}, pc32 vs fp64 "
               f"{worst['pc32']:.2e}, mixed vs fp64 {worst['mixed']:.2e}, vs oracle "
               f"{worst['oracle']:.2e}", flush=True)
     for b, _ in batches.values():
@@ -105,7 +105,7 @@ def test_single_2048_grid_200_steps(gpu_lib):
             worst = max(worst, e)
             assert e <= TOL, f"{name} vs fp64 after {10 * (blk + 1)} steps: {e:.3e}"
         if blk % 5 == 4:
-            print(f"[c2] {10 * (blk + 1)} steps: worst FP32-form vs FP64 {worst:.2e}", flush=True)
+            print(f"[2048] {10 * (blk + 1)} steps: worst FP32-form vs FP64 {worst:.2e}", flush=True)
     for s in sol.values():
         s.close()
 
@@ -158,5 +158,5 @@ def test_serving_trace_from_the_serving_distribution(gpu_lib):
             checked += 1
     assert checked >= 1
     s = sv.summarize(done, wall)
-    print(f"[c5] {s}", flush=True)
+    print(f"[serve] {s}", flush=True)
     assert s["latency_p99_ms"] >= s["latency_p50_ms"] > 0
