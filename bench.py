@@ -371,7 +371,7 @@ def main():
     tuned = None
     if args.autotune:
         import sayram_tune as tu
-        best, bv, trials = tu.tune(probs, prec, steps=1, warmup=1, log=log)
+        best, bv, trials = tu.tune(probs, prec, steps=2, warmup=3, log=log)
         tuned = {"best": best, "value": bv, "trials": len(trials)}
         graph = best["use_graph"] == 1
 

@@ -12,8 +12,8 @@ from conftest import rel_l2
 
 def test_tuner_defaults_are_the_library_defaults():
     o = tu.opts_for(sg.SG_PC32, tu.DEFAULTS)
-    assert list(o.line_tiles) == [16, 160, 16, 768]
-    assert o.check_every == 1 and o.use_graph == 1 and o.streams == 2 and o.fuse_cells == 1024
+    assert list(o.line_tiles) == [16, 160, 16, 768] and list(o.coarse_tiles) == [16, 160, 16, 768]
+    assert o.check_every == 1 and o.use_graph == 1 and o.streams == 1 and o.fuse_cells == 1024
     assert o.group_cells == 1 << 21
     for name, cands in tu.CANDIDATES.items():
         assert tu.DEFAULTS[name] in cands, name
