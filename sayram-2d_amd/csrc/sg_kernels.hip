@@ -1856,15 +1856,19 @@ __global__ void __launch_bounds__(kBlock) k_commit(const InstDev *__restrict__ i
   const InstDev &I = insts[m.inst];
   if (rsc(I)->phase != 1 || rsc(I)->ostatus != ST_DONE) return;   // uniform
   const int K = m.k0 + threadIdx.x;
+  const bool hist = rsc(I)->hist;   // df holds an earlier increment (uniform)
   double v[2] = {0.0, 0.0};
   if (K < I.nx * I.ny) {
     const double xv = gbl(I.x)[K];
     const double fn = (I.noclamp || xv > 0.0) ? xv : 0.0;
     if (I.warm) {
-      const double dold = (double)gbl(I.df)[K], dnew = fn - gbl(I.f)[K];
+      const double dnew = fn - gbl(I.f)[K];
+      if (hist) {
+        const double dold = (double)gbl(I.df)[K];
+        v[0] = dnew * dold;
+        v[1] = dold * dold;
+      }
       gbl(I.df)[K] = (float)dnew;
-      v[0] = dnew * dold;
-      v[1] = dold * dold;
     }
     gbl(I.f)[K] = fn;
   }
