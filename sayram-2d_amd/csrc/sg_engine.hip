@@ -150,9 +150,10 @@ struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
   std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
       gcy[kMaxLev][kNumTileShapes];
-  // cycle: restriction into level l, prolongation into level l, sweeps and
-  // coarsest solves, and every instance with a fused coarse cycle
-  std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev];
+  // cycle: restriction into level l, prolongation into level l, the even
+  // rows of level l (residual after an x-line pre-sweep), sweeps and coarsest
+  // solves, and every instance with a fused coarse cycle
+  std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev], emap[kMaxLev];
   std::vector<LineGrp> sw[kMaxLev][2][2][kNumTileShapes], swx[kMaxLev][kNumTileShapes],
       swy[kMaxLev][kNumTileShapes];
   std::vector<FuseMap> fmap;
@@ -223,8 +224,8 @@ struct DevMaps {   // offsets into one device buffer
       *gcy[kMaxLev][kNumTileShapes];
   int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
       ngcy[kMaxLev][kNumTileShapes];
-  const BlkMap *rmap[kMaxLev], *qmap[kMaxLev];
-  int nrmap[kMaxLev], nqmap[kMaxLev];
+  const BlkMap *rmap[kMaxLev], *qmap[kMaxLev], *emap[kMaxLev];
+  int nrmap[kMaxLev], nqmap[kMaxLev], nemap[kMaxLev];
   const LineGrp *sw[kMaxLev][2][2][kNumTileShapes], *swx[kMaxLev][kNumTileShapes],
       *swy[kMaxLev][kNumTileShapes];
   int nsw[kMaxLev][2][2][kNumTileShapes], nswx[kMaxLev][kNumTileShapes],
@@ -368,7 +369,8 @@ static void apply_defaults(sg_opts &o) {
   if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 1;
-  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 12;   // post-smoothing: x-, then y-lines
+  // an x-line pre-sweep, then post-smoothing with x- and y-lines
+  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 13;
   if (o.coarse_smoother == 0) o.coarse_smoother = o.smoother;
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 1;
@@ -395,8 +397,9 @@ int Batch::init(const sg_opts *o, size_t b) {
   if (opts.streams != 1 && opts.streams != 2) return fail(SG_ERR_ARG, "opts.streams must be 0, 1 or 2");
   if (opts.warm_start < -1 || opts.warm_start > 1)
     return fail(SG_ERR_ARG, "opts.warm_start must be -1, 0 or 1");
-  if (opts.coarse_smoother != opts.smoother && (opts.coarse_smoother & ~12 || !(opts.coarse_smoother & 12)))
-    return fail(SG_ERR_ARG, "opts.coarse_smoother must be 4, 8 or 12 (post-smoothing)");
+  if (opts.coarse_smoother != opts.smoother && (opts.coarse_smoother & ~13 || !(opts.coarse_smoother & 12)))
+    return fail(SG_ERR_ARG, "opts.coarse_smoother must be 4, 8 or 12 (post-smoothing), or one of "
+                            "them plus 1 (an x-line pre-sweep)");
   budget = b;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -636,7 +639,8 @@ int Batch::remove(int slot) {
 // pre-smooths (the fused cycle implements post-smoothing) or when no level is
 // that small. It depends only on the instance and the options.
 int Batch::fuse_level(const InstDev &I) const {
-  if (opts.fuse_cells < 0 || ((opts.smoother | opts.coarse_smoother) & 3)) return I.nlev;
+  // the fused cycle runs post-smoothing, optionally after an x-line pre-sweep
+  if (opts.fuse_cells < 0 || ((opts.smoother | opts.coarse_smoother) & 2)) return I.nlev;
   for (int l = 0; l < I.nlev; ++l)
     if ((long long)I.lev[l].nx * I.lev[l].ny <= opts.fuse_cells) return l;
   return I.nlev;
@@ -680,6 +684,10 @@ int Batch::rebuild() {
       const int nq = prow == 1 ? lx * (ly >> 1) : prow == 2 ? (lx - half_x(lx)) * ly : Nl;
       if (!coarsest && l < lf)
         for (int k0 = 0; k0 < nq; k0 += kBlock) M.qmap[l].push_back({c, k0});
+      if (!coarsest && l < lf && half_residual(l == 0 ? opts.smoother : opts.coarse_smoother)) {
+        const int ne = lx * ((ly + 1) >> 1);
+        for (int k0 = 0; k0 < ne; k0 += kBlock) M.emap[l].push_back({c, k0});
+      }
       if (coarsest) {
         LineGrp g{c, 0, 1, 1};
         auto &set = ly == 1 ? M.gcx[l][tile_shape(lim, lx, 0, l, f64)]
@@ -726,6 +734,7 @@ int Batch::rebuild() {
     xcd_blocks(M.pmap[l], 1);
     xcd_blocks(M.rmap[l], 1);
     xcd_blocks(M.qmap[l], kEnt);   // k_prolong takes kEnt entries per workgroup
+    xcd_blocks(M.emap[l], 1);
   }
   // fused cycles with the most levels first: they are the critical path
   std::stable_sort(M.fmap.begin(), M.fmap.end(), [&](const FuseMap &a, const FuseMap &b) {
@@ -738,6 +747,7 @@ int Batch::rebuild() {
   for (int l = 0; l < kMaxLev; ++l) {
     sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
     sz(M.rmap[l].size(), sizeof(BlkMap)); sz(M.qmap[l].size(), sizeof(BlkMap));
+    sz(M.emap[l].size(), sizeof(BlkMap));
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
@@ -775,6 +785,8 @@ int Batch::rebuild() {
     dmn.nrmap[l] = (int)M.rmap[l].size();
     dmn.qmap[l] = (const BlkMap *)put(M.qmap[l].data(), M.qmap[l].size(), sizeof(BlkMap));
     dmn.nqmap[l] = (int)M.qmap[l].size();
+    dmn.emap[l] = (const BlkMap *)put(M.emap[l].data(), M.emap[l].size(), sizeof(BlkMap));
+    dmn.nemap[l] = (int)M.emap[l].size();
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
@@ -922,8 +934,13 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   }
   if (deeper) {
     tbegin(4);
-    if (!pre_zero) launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, xid, bid);
-    launch_restrict<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, pre_zero, xid, bid);
+    // after a pre-sweep of x-lines alone the residual vanishes on the odd rows
+    const bool even = !pre_zero && half_residual(sm);
+    if (!pre_zero)
+      launch_residual<T>(stream, d_insts, even ? m.emap[l] : m.pmap[l],
+                         even ? m.nemap[l] : m.npmap[l], l, xid, bid, even);
+    launch_restrict<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, pre_zero, even, xid,
+                       bid);
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);
@@ -1162,12 +1179,21 @@ void Batch::count_bytes(const std::vector<int> &part) {
       // a V-cycle sweeps every cell once per direction it smooths: 8
       // (5-point) or 12 (9-point) values per swept cell
       const int sml = l == 0 ? opts.smoother : opts.coarse_smoother;
-      const double dirs = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0);
+      const double dirs = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0) +
+                          ((sml & 1) ? 1.0 : 0.0) + ((sml & 2) ? 1.0 : 0.0);
       t_bytes[l == 0 ? 2 : 3] += vc * dirs * (l == 0 ? 8.0 : 12.0) * T * Nl;
       const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
-      // restriction of b into level l + 1 (x = 0): r and the stored weights
-      // (2 per fine cell on average) in, b out
-      t_bytes[4] += vc * (3.0 * T * Nl + T * Nc);
+      if (!(sml & 3)) {
+        // restriction of b into level l + 1 (x = 0): b and the stored weights
+        // (2 per fine cell on average) in, b out
+        t_bytes[4] += vc * (3.0 * T * Nl + T * Nc);
+      } else {
+        // residual (b, the bands, x; r out) of every cell, or of the even
+        // rows after an x-line pre-sweep, and its restriction (r and the
+        // weights of the cells read; b out)
+        const double share = half_residual(sml) ? 0.5 : 1.0;
+        t_bytes[4] += vc * (share * Nl * (nb + 3.0) * T + share * 3.0 * T * Nl + T * Nc);
+      }
       // prolongation into the lines the first sweep reads (half the cells:
       // the odd rows or columns, 3 stored weights per cell on average), x in
       // and out, and the coarse x once; into every cell (2 weights on

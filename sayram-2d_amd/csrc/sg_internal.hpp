@@ -79,6 +79,12 @@ __host__ __device__ inline int half_x(int nx) { return (nx + 1) >> 1; }
 __host__ __device__ inline int prolong_rows(int smoother) {
   return (smoother & 4) ? 1 : (smoother & 8) ? 2 : 0;
 }
+// After a pre-smoothing sweep of x-lines alone the odd rows (colour 1, solved
+// last from the even rows' new values) satisfy their line equations, so the
+// residual the restriction needs is that of the even rows only.
+__host__ __device__ inline bool half_residual(int smoother) {
+  return (smoother & 3) == 1;
+}
 // position of column i within its row
 __host__ __device__ inline int xslot(int i, int hx) { return (i & 1) ? hx + (i >> 1) : (i >> 1); }
 // column stored at position p of a row

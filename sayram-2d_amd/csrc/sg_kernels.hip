@@ -1073,21 +1073,39 @@ __device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int 
 // r_l = b_l - A_l x_l into the level's scratch (the line-solve buffer cp is
 // free between the pre- and post-smoothing sweeps). Only needed after a
 // pre-smoothing sweep: with x_l = 0 the restriction reads b_l itself.
-template <typename T, int NB>
+// EVEN: after a pre-smoothing sweep of x-lines the odd rows (colour 1, solved
+// last, from the even rows' new values) satisfy their line equations, so the
+// residual vanishes there up to the line solves' rounding; only the even rows
+// are formed (the map covers rows 0, 2, .. compactly), and the restriction
+// reads only those.
+template <int EVEN> __device__ __forceinline__ int residual_count(const Lev &F) {
+  return EVEN ? F.nx * ((F.ny + 1) >> 1) : F.nx * F.ny;
+}
+template <typename T, int NB, bool EVEN>
+__device__ __forceinline__ void residual_point(const InstDev &I, int l, int xid, int bid, int v) {
+  const Lev &F = I.lev[l];
+  int K, i, j;
+  if (EVEN) {
+    const int r = v / F.nx, p = v - r * F.nx;
+    j = 2 * r; i = xcol(p, half_x(F.nx)); K = j * F.nx + p;
+  } else {
+    K = v;
+    cell_ij(K, F.nx, half_x(F.nx), i, j);
+  }
+  T *x; const T *b;
+  level_xb<T>(I, l, xid, bid, x, b);
+  ptr<T>(F.cp)[K] = b[K] - stencil_apply<T, NB>(F, K, i, j, x);
+}
+template <typename T, int NB, bool EVEN>
 __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
-  const Lev &F = I.lev[l];
-  const int K = m.k0 + threadIdx.x;
-  if (K >= F.nx * F.ny) return;
-  int i, j;
-  cell_ij(K, F.nx, half_x(F.nx), i, j);
-  T *x; const T *b;
-  level_xb<T>(I, l, xid, bid, x, b);
-  ptr<T>(F.cp)[K] = b[K] - stencil_apply<T, NB>(F, K, i, j, x);
+  const int v = m.k0 + threadIdx.x;
+  if (v >= residual_count<EVEN>(I.lev[l])) return;
+  residual_point<T, NB, EVEN>(I, l, xid, bid, v);
 }
 
 // b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J).
@@ -1097,7 +1115,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
 // consecutive in the fine row's even and odd halves, so every load is
 // contiguous across the wave; the coarse store alternates between the two
 // contiguous halves of the coarse row.
-template <typename T, bool FROM_B>
+template <typename T, bool FROM_B, bool EVEN = false>
 __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid, int bid, int KC) {
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
@@ -1109,11 +1127,13 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
   const auto pw0 = gbl(cptr<T>(F.pw[0])), pw1 = gbl(cptr<T>(F.pw[1]));
   const auto pw2 = gbl(cptr<T>(F.pw[2])), pw3 = gbl(cptr<T>(F.pw[3]));
   // All 18 loads are issued before the sum (fine points outside the grid are
-  // clamped in and weighted by zero), then added in row-major order.
+  // clamped in and weighted by zero), then added in row-major order. EVEN:
+  // the residual is zero on the odd rows 2J +- 1, so only row 2J is read.
+  constexpr int db0 = EVEN ? 0 : -1, db1 = EVEN ? 0 : 1;
   T wv[3][3], rv[3][3];
   bool ok[3][3];
 #pragma unroll
-  for (int db = -1; db <= 1; ++db) {
+  for (int db = db0; db <= db1; ++db) {
     const int fj = 2 * J + db;
     const bool okj = fj >= 0 && fj < F.ny;
     const size_t rowf = (size_t)(okj ? fj : 2 * J) * F.nx;
@@ -1134,13 +1154,13 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
   }
   T s = T(0);
 #pragma unroll
-  for (int b = 0; b < 3; ++b)
+  for (int b = db0 + 1; b <= db1 + 1; ++b)
 #pragma unroll
     for (int a = 0; a < 3; ++a) s += (ok[b][a] ? wv[b][a] : T(0)) * rv[b][a];
   gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
 }
 
-template <typename T, bool FROM_B>
+template <typename T, bool FROM_B, bool EVEN>
 __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
@@ -1149,7 +1169,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   if (rsc(I)->status != ST_ACTIVE) return;
   const int KC = m.k0 + threadIdx.x;
   if (KC >= I.lev[l + 1].nx * I.lev[l + 1].ny) return;
-  restrict_point<T, FROM_B>(I, l, xid, bid, KC);
+  restrict_point<T, FROM_B, EVEN>(I, l, xid, bid, KC);
 }
 
 // x_l += P x_{l+1}. ZERO: x_l = P x_{l+1}.
@@ -1296,9 +1316,11 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const InstDev *__restrict__ 
 // Fused coarse cycle: the levels of one instance from `lev` (its first level
 // with at most kFuseCells cells) down to its coarsest and back, in one
 // workgroup with barriers between the phases, instead of several launches per
-// level. The cycle is the per-level one with post-smoothing only: restriction
-// of b to the coarsest level, its exact line solve, then per level the
-// prolongation of the lines the first sweep reads and the zebra sweeps.
+// level. The cycle is the per-level one: per level on the way down an
+// optional pre-smoothing sweep of x-lines and the restriction of the
+// residual (of b itself without pre-smoothing), the exact line solve of the
+// coarsest level, then per level the prolongation of the lines the first
+// post-sweep reads and the zebra post-sweeps.
 //
 // A level being smoothed lives in LDS (x, and per sweep the line right-hand
 // sides and factors), so only the parallel phases touch global memory: every
@@ -1488,7 +1510,7 @@ __device__ void fused_smooth(const InstDev &I, int l, int xid, int bid, bool pos
 }
 
 // smoother: the finest level's schedule in bits 0-3, the coarser levels' in
-// bits 4-7 (post-smoothing bits only)
+// bits 4-7 (post-smoothing bits and the x-line pre-smoothing bit)
 template <typename T, int CAP, int SCAP>
 __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__restrict__ insts,
                                                                const FuseMap *__restrict__ map,
@@ -1499,9 +1521,33 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
   const InstDev &I = insts[fm.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
   const int lf = fm.lev, nl = I.nlev;
-  for (int m = lf; m + 1 < nl; ++m) {   // b_{m+1} = R b_m (x_m = 0)
+  auto sched = [&](int m) { return m == 0 ? (smoother & 15) : (smoother >> 4); };
+  for (int m = lf; m + 1 < nl; ++m) {
     const int nc = I.lev[m + 1].nx * I.lev[m + 1].ny;
-    for (int KC = threadIdx.x; KC < nc; KC += blockDim.x) restrict_point<T, true>(I, m, xid, bid, KC);
+    if (sched(m) & 1) {
+      // pre-smoothing with x-lines from x_m = 0 (colour 0 from b alone), x_m
+      // kept for the ascent, then b_{m+1} = R r_m with r_m = b_m - A_m x_m,
+      // which is zero on the odd rows (solved last)
+      if (m == 0) {
+        fused_sweep<T, 5, 0, false>(I, m, bid, 0, true, S, G);
+        fused_sweep<T, 5, 0, false>(I, m, bid, 1, false, S, G);
+      } else {
+        fused_sweep<T, 9, 0, false>(I, m, bid, 0, true, S, G);
+        fused_sweep<T, 9, 0, false>(I, m, bid, 1, false, S, G);
+      }
+      fused_x<T>(I, m, xid, S, false);
+      const int nr = residual_count<true>(I.lev[m]);
+      for (int v = threadIdx.x; v < nr; v += blockDim.x) {
+        if (m == 0) residual_point<T, 5, true>(I, m, xid, bid, v);
+        else residual_point<T, 9, true>(I, m, xid, bid, v);
+      }
+      __syncthreads();
+      for (int KC = threadIdx.x; KC < nc; KC += blockDim.x)
+        restrict_point<T, false, true>(I, m, xid, bid, KC);
+    } else {   // b_{m+1} = R b_m (x_m = 0)
+      for (int KC = threadIdx.x; KC < nc; KC += blockDim.x)
+        restrict_point<T, true>(I, m, xid, bid, KC);
+    }
     __syncthreads();
   }
   {   // exact solve of the coarsest level (x = 0): one row or one column
@@ -1515,21 +1561,24 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
     }
     fused_x<T>(I, c, xid, S, false);
   }
-  for (int m = nl - 2; m >= lf; --m) {  // x_m = P x_{m+1}, then post-smoothing
+  for (int m = nl - 2; m >= lf; --m) {  // x_m (+)= P x_{m+1}, then post-smoothing
     const Lev &F = I.lev[m];
     // schedule of level m: the finest level's (low 4 bits) or the coarser ones'
-    const int sml = m == 0 ? (smoother & 15) : (smoother >> 4);
+    const int sml = sched(m);
     const bool post_x = sml & 4, post_y = sml & 8;
     const int rows = prolong_rows(sml);
-    if (rows == 1) {
-      const int np = prolong_count<1>(F);
-      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 1>(I, m, xid, k);
-    } else if (rows == 2) {
-      const int np = prolong_count<2>(F);
-      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 2>(I, m, xid, k);
-    } else {
-      const int np = prolong_count<0>(F);
-      for (int k = threadIdx.x; k < np; k += blockDim.x) prolong_point<T, true, 0>(I, m, xid, k);
+    const int np = rows == 1 ? prolong_count<1>(F) : rows == 2 ? prolong_count<2>(F)
+                                                                : prolong_count<0>(F);
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+      if (sml & 1) {   // pre-smoothed: the correction is added to x_m
+        if (rows == 1) prolong_point<T, false, 1>(I, m, xid, k);
+        else if (rows == 2) prolong_point<T, false, 2>(I, m, xid, k);
+        else prolong_point<T, false, 0>(I, m, xid, k);
+      } else {
+        if (rows == 1) prolong_point<T, true, 1>(I, m, xid, k);
+        else if (rows == 2) prolong_point<T, true, 2>(I, m, xid, k);
+        else prolong_point<T, true, 0>(I, m, xid, k);
+      }
     }
     __syncthreads();
     if (m == 0) fused_smooth<T, 5>(I, m, xid, bid, post_x, post_y, S, G);
@@ -2240,27 +2289,33 @@ template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, 
 
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
-                     int bid) {
+                     int bid, bool even) {
   if (!nb) return;
-  if (l == 0) k_residual<T, 5><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  else k_residual<T, 9><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  if (even) {
+    if (l == 0) k_residual<T, 5, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    else k_residual<T, 9, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  } else {
+    if (l == 0) k_residual<T, 5, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+    else k_residual<T, 9, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  }
 }
 template void launch_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
-                                      int);
+                                      int, bool);
 template void launch_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
-                                     int);
+                                     int, bool);
 
 template <typename T>
 void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
-                     int xid, int bid) {
+                     bool even, int xid, int bid) {
   if (!nb) return;
-  if (from_b) k_restrict<T, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  else k_restrict<T, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  if (from_b) k_restrict<T, true, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else if (even) k_restrict<T, false, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_restrict<T, false, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
 }
 template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                      int, int);
+                                      bool, int, int);
 template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                     int, int);
+                                     bool, int, int);
 
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,

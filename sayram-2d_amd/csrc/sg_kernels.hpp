@@ -15,13 +15,16 @@ void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int n
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  int shape, bool zero, int xid, int bid);
+// even: the residual of the even rows only (after a pre-smoothing sweep of
+// x-lines it vanishes on the odd rows); m then covers rows 0, 2, .. compactly
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
-                     int bid);
-// from_b: restrict the level's right-hand side (x_l = 0) instead of its residual
+                     int bid, bool even);
+// from_b: restrict the level's right-hand side (x_l = 0) instead of its
+// residual; even: the residual is zero on the odd rows
 template <typename T>
 void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
-                     int xid, int bid);
+                     bool even, int xid, int bid);
 // rows: 0 every cell, 1 odd rows only, 2 odd columns only (the cells the first
 // post-smoothing sweep reads)
 template <typename T>

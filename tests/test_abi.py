@@ -66,7 +66,7 @@ def test_option_errors_come_before_any_device_call():
         sg.Batch(sg.make_opts(fuse_cells=2048))          # FP64 V-cycles: at most 1024
     assert "fuse_cells" in str(e.value)
     for opts in (sg.make_opts(precision=sg.SG_PC32, fuse_cells=4097),
-                 sg.make_opts(coarse_smoother=3), sg.make_opts(coarse_smoother=5)):
+                 sg.make_opts(coarse_smoother=3), sg.make_opts(coarse_smoother=6)):
         with pytest.raises(sg.SayramError) as e:
             sg.Batch(opts)
         assert e.value.code == sg.SG_ERR_ARG

@@ -209,3 +209,22 @@ def test_two_groups_equal_one_group_bitwise(gpu_lib):
         assert np.array_equal(out[-1][0][k], out[5000][0][k])
     assert out[-1][1] == out[5000][1]
     assert [x[0] for x in out[5000][1]] == [4, 3, 5, 3, 4]
+
+
+@pytest.mark.parametrize("smoother", [12, 13, 15, 5])
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_smoother_schedules_stay_on_the_oracle(gpu_lib, smoother, precision):
+    """Post-smoothing alone (12), an x-line pre-sweep before it (13, the
+    default: the fused cycle then pre-smooths too, and only the even rows'
+    residual is restricted), pre-sweeps in both directions (15: every
+    residual formed) and x-lines only (5): different preconditioners, the same
+    solution to the solve tolerance, in every precision."""
+    probs = [sc.seeded_case(nx, ny, seed=160 + k, scale=s) for k, (nx, ny, s) in
+             enumerate([(70, 45, 0.5), (33, 130, 2.0), (120, 96, 1.0), (9, 300, 3.0)])]
+    b = gpu_lib.Batch(gpu_lib.make_opts(precision=precision, smoother=smoother))
+    slots = [b.add(p) for p in probs]
+    b.step(3)
+    for p, sl in zip(probs, slots):
+        assert rel_l2(b.f(sl), orc.run(orc.OracleProblem(p), p.f0, 3)) <= TOL
+        assert b.stats(sl)["status"] == 0
+    b.close()
