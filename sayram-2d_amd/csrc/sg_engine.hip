@@ -1179,9 +1179,13 @@ void Batch::count_bytes(const std::vector<int> &part) {
       // a V-cycle sweeps every cell once per direction it smooths: 8
       // (5-point) or 12 (9-point) values per swept cell
       const int sml = l == 0 ? opts.smoother : opts.coarse_smoother;
-      const double dirs = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0) +
-                          ((sml & 1) ? 1.0 : 0.0) + ((sml & 2) ? 1.0 : 0.0);
-      t_bytes[l == 0 ? 2 : 3] += vc * dirs * (l == 0 ? 8.0 : 12.0) * T * Nl;
+      const double vals = l == 0 ? 8.0 : 12.0;
+      const double post = ((sml & 4) ? 1.0 : 0.0) + ((sml & 8) ? 1.0 : 0.0);
+      const double pre = ((sml & 1) ? 1.0 : 0.0) + ((sml & 2) ? 1.0 : 0.0);
+      // the first colour of the first pre-sweep starts from x = 0: b, the
+      // lower band and the two factors in, x out (5 values per swept cell)
+      const double pre_vals = pre > 0 ? (5.0 + vals) / 2.0 + (pre - 1.0) * vals : 0.0;
+      t_bytes[l == 0 ? 2 : 3] += vc * (post * vals + pre_vals) * T * Nl;
       const double Nc = (double)I.lev[l + 1].nx * I.lev[l + 1].ny;
       if (!(sml & 3)) {
         // restriction of b into level l + 1 (x = 0): b and the stored weights
