@@ -150,10 +150,9 @@ struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
   std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
       gcy[kMaxLev][kNumTileShapes];
-  // cycle: restriction into level l, prolongation into level l, the even
-  // rows of level l (residual after an x-line pre-sweep), sweeps and coarsest
-  // solves, and every instance with a fused coarse cycle
-  std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev], emap[kMaxLev];
+  // cycle: restriction into level l, prolongation into level l, sweeps and
+  // coarsest solves, and every instance with a fused coarse cycle
+  std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev];
   std::vector<LineGrp> sw[kMaxLev][2][2][kNumTileShapes], swx[kMaxLev][kNumTileShapes],
       swy[kMaxLev][kNumTileShapes];
   std::vector<FuseMap> fmap;
@@ -224,8 +223,8 @@ struct DevMaps {   // offsets into one device buffer
       *gcy[kMaxLev][kNumTileShapes];
   int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
       ngcy[kMaxLev][kNumTileShapes];
-  const BlkMap *rmap[kMaxLev], *qmap[kMaxLev], *emap[kMaxLev];
-  int nrmap[kMaxLev], nqmap[kMaxLev], nemap[kMaxLev];
+  const BlkMap *rmap[kMaxLev], *qmap[kMaxLev];
+  int nrmap[kMaxLev], nqmap[kMaxLev];
   const LineGrp *sw[kMaxLev][2][2][kNumTileShapes], *swx[kMaxLev][kNumTileShapes],
       *swy[kMaxLev][kNumTileShapes];
   int nsw[kMaxLev][2][2][kNumTileShapes], nswx[kMaxLev][kNumTileShapes],
@@ -684,10 +683,6 @@ int Batch::rebuild() {
       const int nq = prow == 1 ? lx * (ly >> 1) : prow == 2 ? (lx - half_x(lx)) * ly : Nl;
       if (!coarsest && l < lf)
         for (int k0 = 0; k0 < nq; k0 += kBlock) M.qmap[l].push_back({c, k0});
-      if (!coarsest && l < lf && half_residual(l == 0 ? opts.smoother : opts.coarse_smoother)) {
-        const int ne = lx * ((ly + 1) >> 1);
-        for (int k0 = 0; k0 < ne; k0 += kBlock) M.emap[l].push_back({c, k0});
-      }
       if (coarsest) {
         LineGrp g{c, 0, 1, 1};
         auto &set = ly == 1 ? M.gcx[l][tile_shape(lim, lx, 0, l, f64)]
@@ -734,7 +729,6 @@ int Batch::rebuild() {
     xcd_blocks(M.pmap[l], 1);
     xcd_blocks(M.rmap[l], 1);
     xcd_blocks(M.qmap[l], kEnt);   // k_prolong takes kEnt entries per workgroup
-    xcd_blocks(M.emap[l], 1);
   }
   // fused cycles with the most levels first: they are the critical path
   std::stable_sort(M.fmap.begin(), M.fmap.end(), [&](const FuseMap &a, const FuseMap &b) {
@@ -747,7 +741,6 @@ int Batch::rebuild() {
   for (int l = 0; l < kMaxLev; ++l) {
     sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
     sz(M.rmap[l].size(), sizeof(BlkMap)); sz(M.qmap[l].size(), sizeof(BlkMap));
-    sz(M.emap[l].size(), sizeof(BlkMap));
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
@@ -785,8 +778,6 @@ int Batch::rebuild() {
     dmn.nrmap[l] = (int)M.rmap[l].size();
     dmn.qmap[l] = (const BlkMap *)put(M.qmap[l].data(), M.qmap[l].size(), sizeof(BlkMap));
     dmn.nqmap[l] = (int)M.qmap[l].size();
-    dmn.emap[l] = (const BlkMap *)put(M.emap[l].data(), M.emap[l].size(), sizeof(BlkMap));
-    dmn.nemap[l] = (int)M.emap[l].size();
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       for (int d = 0; d < 2; ++d)
         for (int p = 0; p < 2; ++p) {
@@ -934,13 +925,14 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   }
   if (deeper) {
     tbegin(4);
-    // after a pre-sweep of x-lines alone the residual vanishes on the odd rows
-    const bool even = !pre_zero && half_residual(sm);
-    if (!pre_zero)
-      launch_residual<T>(stream, d_insts, even ? m.emap[l] : m.pmap[l],
-                         even ? m.nemap[l] : m.npmap[l], l, xid, bid, even);
-    launch_restrict<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, pre_zero, even, xid,
-                       bid);
+    // after a pre-sweep of x-lines alone the residual vanishes on the odd
+    // rows: the even rows' residual is formed inside the restriction
+    if (!pre_zero && half_residual(sm)) {
+      launch_restrict_residual<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, xid, bid);
+    } else {
+      if (!pre_zero) launch_residual<T>(stream, d_insts, m.pmap[l], m.npmap[l], l, xid, bid);
+      launch_restrict<T>(stream, d_insts, m.rmap[l + 1], m.nrmap[l + 1], l, pre_zero, xid, bid);
+    }
     tend(4);
     vcycle<T>(l + 1, xid, bid);
     tbegin(5);

@@ -1033,18 +1033,19 @@ __device__ __forceinline__ T stencil_apply(const Lev &L, size_t K, int i, int j,
     asw = band(BSW); ase = band(BSE); anw = band(BNW); ane = band(BNE);
     xsw = x[kw - ds]; xse = x[ke - ds]; xnw = x[kw + dn]; xne = x[ke + dn];
   }
+  // explicit fused multiply-adds, so that every caller rounds alike
   T s = ac * x0;
-  s += (hw ? aw : T(0)) * xw;
-  s += (he ? ae : T(0)) * xe;
-  s += (hs ? as : T(0)) * xs;
+  s = fma(hw ? aw : T(0), xw, s);
+  s = fma(he ? ae : T(0), xe, s);
+  s = fma(hs ? as : T(0), xs, s);
   if (NB == 9) {
-    s += (hs && hw ? asw : T(0)) * xsw;
-    s += (hs && he ? ase : T(0)) * xse;
+    s = fma(hs && hw ? asw : T(0), xsw, s);
+    s = fma(hs && he ? ase : T(0), xse, s);
   }
-  s += (hn ? an : T(0)) * xn;
+  s = fma(hn ? an : T(0), xn, s);
   if (NB == 9) {
-    s += (hn && hw ? anw : T(0)) * xnw;
-    s += (hn && he ? ane : T(0)) * xne;
+    s = fma(hn && hw ? anw : T(0), xnw, s);
+    s = fma(hn && he ? ane : T(0), xne, s);
   }
   return s;
 }
@@ -1071,41 +1072,24 @@ __device__ __forceinline__ double spmv64(const InstDev &I, size_t K, int i, int 
 }
 
 // r_l = b_l - A_l x_l into the level's scratch (the line-solve buffer cp is
-// free between the pre- and post-smoothing sweeps). Only needed after a
-// pre-smoothing sweep: with x_l = 0 the restriction reads b_l itself.
-// EVEN: after a pre-smoothing sweep of x-lines the odd rows (colour 1, solved
-// last, from the even rows' new values) satisfy their line equations, so the
-// residual vanishes there up to the line solves' rounding; only the even rows
-// are formed (the map covers rows 0, 2, .. compactly), and the restriction
-// reads only those.
-template <int EVEN> __device__ __forceinline__ int residual_count(const Lev &F) {
-  return EVEN ? F.nx * ((F.ny + 1) >> 1) : F.nx * F.ny;
-}
-template <typename T, int NB, bool EVEN>
-__device__ __forceinline__ void residual_point(const InstDev &I, int l, int xid, int bid, int v) {
-  const Lev &F = I.lev[l];
-  int K, i, j;
-  if (EVEN) {
-    const int r = v / F.nx, p = v - r * F.nx;
-    j = 2 * r; i = xcol(p, half_x(F.nx)); K = j * F.nx + p;
-  } else {
-    K = v;
-    cell_ij(K, F.nx, half_x(F.nx), i, j);
-  }
-  T *x; const T *b;
-  level_xb<T>(I, l, xid, bid, x, b);
-  ptr<T>(F.cp)[K] = b[K] - stencil_apply<T, NB>(F, K, i, j, x);
-}
-template <typename T, int NB, bool EVEN>
+// free between the pre- and post-smoothing sweeps), after a pre-smoothing
+// sweep: with x_l = 0 the restriction reads b_l itself, and after an x-line
+// pre-sweep alone restrict_residual_point forms the residual it needs.
+template <typename T, int NB>
 __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
   const BlkMap m = map[blockIdx.x];
   const InstDev &I = insts[m.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
-  const int v = m.k0 + threadIdx.x;
-  if (v >= residual_count<EVEN>(I.lev[l])) return;
-  residual_point<T, NB, EVEN>(I, l, xid, bid, v);
+  const Lev &F = I.lev[l];
+  const int K = m.k0 + threadIdx.x;
+  if (K >= F.nx * F.ny) return;
+  int i, j;
+  cell_ij(K, F.nx, half_x(F.nx), i, j);
+  T *x; const T *b;
+  level_xb<T>(I, l, xid, bid, x, b);
+  ptr<T>(F.cp)[K] = b[K] - stencil_apply<T, NB>(F, K, i, j, x);
 }
 
 // b_{l+1}(C) = sum_f P(f, C) r_l(f) over the 3x3 fine block around (2I, 2J).
@@ -1115,7 +1099,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(const InstDev *__restrict__
 // consecutive in the fine row's even and odd halves, so every load is
 // contiguous across the wave; the coarse store alternates between the two
 // contiguous halves of the coarse row.
-template <typename T, bool FROM_B, bool EVEN = false>
+template <typename T, bool FROM_B>
 __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid, int bid, int KC) {
   const Lev &F = I.lev[l];
   const Lev &Cl = I.lev[l + 1];
@@ -1127,13 +1111,11 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
   const auto pw0 = gbl(cptr<T>(F.pw[0])), pw1 = gbl(cptr<T>(F.pw[1]));
   const auto pw2 = gbl(cptr<T>(F.pw[2])), pw3 = gbl(cptr<T>(F.pw[3]));
   // All 18 loads are issued before the sum (fine points outside the grid are
-  // clamped in and weighted by zero), then added in row-major order. EVEN:
-  // the residual is zero on the odd rows 2J +- 1, so only row 2J is read.
-  constexpr int db0 = EVEN ? 0 : -1, db1 = EVEN ? 0 : 1;
+  // clamped in and weighted by zero), then added in row-major order.
   T wv[3][3], rv[3][3];
   bool ok[3][3];
 #pragma unroll
-  for (int db = db0; db <= db1; ++db) {
+  for (int db = -1; db <= 1; ++db) {
     const int fj = 2 * J + db;
     const bool okj = fj >= 0 && fj < F.ny;
     const size_t rowf = (size_t)(okj ? fj : 2 * J) * F.nx;
@@ -1154,13 +1136,63 @@ __device__ __forceinline__ void restrict_point(const InstDev &I, int l, int xid,
   }
   T s = T(0);
 #pragma unroll
-  for (int b = db0 + 1; b <= db1 + 1; ++b)
+  for (int b = 0; b < 3; ++b)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) s += (ok[b][a] ? wv[b][a] : T(0)) * rv[b][a];
+    for (int a = 0; a < 3; ++a) s = fma(ok[b][a] ? wv[b][a] : T(0), rv[b][a], s);
   gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
 }
 
-template <typename T, bool FROM_B, bool EVEN>
+// b_{l+1}(C) = sum_f P(f, C) r_l(f) after an x-line pre-sweep: r_l vanishes
+// on the odd rows, and the three fine points of row 2 J form theirs here,
+// r = b - A x as residual_point does (same operations), instead of reading a
+// stored residual: one launch per level and no residual array written or
+// read. The odd fine points are shared by two coarse points, so they are
+// formed twice.
+template <typename T, int NB>
+__device__ __forceinline__ void restrict_residual_point(const InstDev &I, int l, int xid, int bid,
+                                                        int KC) {
+  const Lev &F = I.lev[l];
+  const Lev &Cl = I.lev[l + 1];
+  const int J = KC / Cl.nx, Ic = KC - J * Cl.nx;    // column order
+  const int hxf = half_x(F.nx);
+  T *x; const T *b;
+  level_xb<T>(I, l, xid, bid, x, b);
+  const auto pw0 = gbl(cptr<T>(F.pw[0])), pw1 = gbl(cptr<T>(F.pw[1]));
+  const int fj = 2 * J;
+  const size_t rowf = (size_t)fj * F.nx;
+  T wv[3], rv[3];
+  bool ok[3];
+#pragma unroll
+  for (int da = -1; da <= 1; ++da) {
+    const int fi = 2 * Ic + da;
+    const bool oki = fi >= 0 && fi < F.nx;
+    const int ci = oki ? fi : 2 * Ic;
+    const size_t kf = rowf + xslot(ci, hxf);
+    // parent slot of C: 1 for the point to the west (da < 0), 0 otherwise;
+    // the point at C itself is injected with weight exactly 1
+    wv[da + 1] = da == 0 ? T(1) : T(da < 0 ? pw1[kf] : pw0[kf]);
+    rv[da + 1] = gbl(b)[kf] - stencil_apply<T, NB>(F, kf, ci, fj, x);
+    ok[da + 1] = oki;
+  }
+  T s = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s = fma(ok[a] ? wv[a] : T(0), rv[a], s);
+  gbl(ptr<T>(Cl.b))[(size_t)J * Cl.nx + xslot(Ic, half_x(Cl.nx))] = s;
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(kBlock) k_restrict_residual(const InstDev *__restrict__ insts,
+                                                             const BlkMap *__restrict__ map, int l,
+                                                             int xid, int bid) {
+  const BlkMap m = map[blockIdx.x];
+  const InstDev &I = insts[m.inst];
+  if (rsc(I)->status != ST_ACTIVE) return;
+  const int KC = m.k0 + threadIdx.x;
+  if (KC >= I.lev[l + 1].nx * I.lev[l + 1].ny) return;
+  restrict_residual_point<T, NB>(I, l, xid, bid, KC);
+}
+
+template <typename T, bool FROM_B>
 __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__ insts,
                                                     const BlkMap *__restrict__ map, int l,
                                                     int xid, int bid) {
@@ -1169,7 +1201,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const InstDev *__restrict__
   if (rsc(I)->status != ST_ACTIVE) return;
   const int KC = m.k0 + threadIdx.x;
   if (KC >= I.lev[l + 1].nx * I.lev[l + 1].ny) return;
-  restrict_point<T, FROM_B, EVEN>(I, l, xid, bid, KC);
+  restrict_point<T, FROM_B>(I, l, xid, bid, KC);
 }
 
 // x_l += P x_{l+1}. ZERO: x_l = P x_{l+1}.
@@ -1536,14 +1568,10 @@ __global__ void __launch_bounds__(kFuseThreads) k_coarse_cycle(const InstDev *__
         fused_sweep<T, 9, 0, false>(I, m, bid, 1, false, S, G);
       }
       fused_x<T>(I, m, xid, S, false);
-      const int nr = residual_count<true>(I.lev[m]);
-      for (int v = threadIdx.x; v < nr; v += blockDim.x) {
-        if (m == 0) residual_point<T, 5, true>(I, m, xid, bid, v);
-        else residual_point<T, 9, true>(I, m, xid, bid, v);
+      for (int KC = threadIdx.x; KC < nc; KC += blockDim.x) {
+        if (m == 0) restrict_residual_point<T, 5>(I, m, xid, bid, KC);
+        else restrict_residual_point<T, 9>(I, m, xid, bid, KC);
       }
-      __syncthreads();
-      for (int KC = threadIdx.x; KC < nc; KC += blockDim.x)
-        restrict_point<T, false, true>(I, m, xid, bid, KC);
     } else {   // b_{m+1} = R b_m (x_m = 0)
       for (int KC = threadIdx.x; KC < nc; KC += blockDim.x)
         restrict_point<T, true>(I, m, xid, bid, KC);
@@ -2289,33 +2317,39 @@ template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, 
 
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
-                     int bid, bool even) {
+                     int bid) {
   if (!nb) return;
-  if (even) {
-    if (l == 0) k_residual<T, 5, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_residual<T, 9, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  } else {
-    if (l == 0) k_residual<T, 5, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-    else k_residual<T, 9, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  }
+  if (l == 0) k_residual<T, 5><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_residual<T, 9><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
 }
 template void launch_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
-                                      int, bool);
+                                      int);
 template void launch_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, int,
-                                     int, bool);
+                                     int);
+
+template <typename T>
+void launch_restrict_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l,
+                              int xid, int bid) {
+  if (!nb) return;
+  if (l == 0) k_restrict_residual<T, 5><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_restrict_residual<T, 9><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+}
+template void launch_restrict_residual<double>(hipStream_t, const InstDev *, const BlkMap *, int,
+                                               int, int, int);
+template void launch_restrict_residual<float>(hipStream_t, const InstDev *, const BlkMap *, int,
+                                              int, int, int);
 
 template <typename T>
 void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
-                     bool even, int xid, int bid) {
+                     int xid, int bid) {
   if (!nb) return;
-  if (from_b) k_restrict<T, true, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  else if (even) k_restrict<T, false, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
-  else k_restrict<T, false, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  if (from_b) k_restrict<T, true><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
+  else k_restrict<T, false><<<nb, kBlock, 0, s>>>(d, m, l, xid, bid);
 }
 template void launch_restrict<double>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                      bool, int, int);
+                                      int, int);
 template void launch_restrict<float>(hipStream_t, const InstDev *, const BlkMap *, int, int, bool,
-                                     bool, int, int);
+                                     int, int);
 
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,

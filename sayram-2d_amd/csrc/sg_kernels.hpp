@@ -15,23 +15,27 @@ void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int n
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  int shape, bool zero, int xid, int bid);
-// even: the residual of the even rows only (after a pre-smoothing sweep of
-// x-lines it vanishes on the odd rows); m then covers rows 0, 2, .. compactly
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
-                     int bid, bool even);
+                     int bid);
+// the residual of the even rows formed and restricted in one pass (after an
+// x-line pre-sweep the residual of the odd rows vanishes)
+template <typename T>
+void launch_restrict_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l,
+                              int xid, int bid);
 // from_b: restrict the level's right-hand side (x_l = 0) instead of its
-// residual; even: the residual is zero on the odd rows
+// residual
 template <typename T>
 void launch_restrict(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool from_b,
-                     bool even, int xid, int bid);
+                     int xid, int bid);
 // rows: 0 every cell, 1 odd rows only, 2 odd columns only (the cells the first
 // post-smoothing sweep reads)
 template <typename T>
 void launch_prolong(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, bool zero,
                     int xid, int rows);
-// the fused coarse cycle of the instances in m (one workgroup each); post-
-// smoothing schedules only (smoother bits 4 and 8)
+// the fused coarse cycle of the instances in m (one workgroup each): post-
+// smoothing in x and/or y (smoother bits 4 and 8), optionally after an x-line
+// pre-sweep (bit 1); a y-line pre-sweep (bit 2) is not fused
 // cap: the largest level a fused instance may have (fuse_cells); it selects
 // the LDS layout of the kernel
 template <typename T>
