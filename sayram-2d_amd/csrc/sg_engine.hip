@@ -1184,11 +1184,14 @@ void Batch::count_bytes(const std::vector<int> &part) {
         // (2 per fine cell on average) in, b out
         t_bytes[4] += vc * (3.0 * T * Nl + T * Nc);
       } else {
-        // residual (b, the bands, x; r out) of every cell, or of the even
-        // rows after an x-line pre-sweep, and its restriction (r and the
-        // weights of the cells read; b out)
-        const double share = half_residual(sml) ? 0.5 : 1.0;
-        t_bytes[4] += vc * (share * Nl * (nb + 3.0) * T + share * 3.0 * T * Nl + T * Nc);
+        // after an x-line pre-sweep: the even rows' residual formed inside
+        // the restriction (b, the bands, x and 2 stored weights per cell in;
+        // b out); otherwise the residual of every cell (b, the bands, x; r
+        // out) and its restriction (r and 2 weights per cell in; b out)
+        if (half_residual(sml))
+          t_bytes[4] += vc * (0.5 * Nl * (nb + 4.0) * T + T * Nc);
+        else
+          t_bytes[4] += vc * (Nl * (nb + 3.0) * T + 3.0 * T * Nl + T * Nc);
       }
       // prolongation into the lines the first sweep reads (half the cells:
       // the odd rows or columns, 3 stored weights per cell on average), x in
