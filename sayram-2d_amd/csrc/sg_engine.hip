@@ -145,10 +145,12 @@ struct Slot {
 
 // Setup maps cover every level of every instance (assembly, Galerkin chain,
 // line factors). The cycle maps leave out the levels an instance runs in the
-// fused coarse cycle.
+// fused coarse cycle. The factor maps gfac hold the line groups of both
+// colours (a factorisation does not depend on x, so both colours of a
+// direction and tile shape go in one launch).
 struct Maps {
   std::vector<BlkMap> cmap[kMaxLev], pmap[kMaxLev];
-  std::vector<LineGrp> gmap[kMaxLev][2][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
+  std::vector<LineGrp> gfac[kMaxLev][2][kNumTileShapes], gcx[kMaxLev][kNumTileShapes],
       gcy[kMaxLev][kNumTileShapes];
   // cycle: restriction into level l, prolongation into level l, sweeps and
   // coarsest solves, and every instance with a fused coarse cycle
@@ -219,9 +221,9 @@ static void xcd_blocks(std::vector<BlkMap> &v, int unit) {
 struct DevMaps {   // offsets into one device buffer
   const BlkMap *cmap[kMaxLev], *pmap[kMaxLev];
   int ncmap[kMaxLev], npmap[kMaxLev];
-  const LineGrp *gmap[kMaxLev][2][2][kNumTileShapes], *gcx[kMaxLev][kNumTileShapes],
+  const LineGrp *gfac[kMaxLev][2][kNumTileShapes], *gcx[kMaxLev][kNumTileShapes],
       *gcy[kMaxLev][kNumTileShapes];
-  int ngmap[kMaxLev][2][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
+  int ngfac[kMaxLev][2][kNumTileShapes], ngcx[kMaxLev][kNumTileShapes],
       ngcy[kMaxLev][kNumTileShapes];
   const BlkMap *rmap[kMaxLev], *qmap[kMaxLev];
   int nrmap[kMaxLev], nqmap[kMaxLev];
@@ -700,7 +702,7 @@ int Batch::rebuild() {
           for (int p = 0; p < 2; ++p)
             for (int f = p; f < nl; f += 2 * per) {
               const int cntl = std::min(per, (nl - f + 1) / 2);
-              M.gmap[l][d][p][sh].push_back({c, f, cntl, 2});
+              M.gfac[l][d][sh].push_back({c, f, cntl, 2});
               if (l < lf) M.sw[l][d][p][sh].push_back({c, f, cntl, 2});
             }
         }
@@ -710,20 +712,23 @@ int Batch::rebuild() {
   // Longest lines first: a group walks its lines tile by tile, so the groups
   // of the longest lines are the critical path of a sweep and must start at
   // once; the short groups of other instances fill the machine around them.
+  // The column groups of one colour are neighbours in memory (one half of
+  // each row of the split layout), and xcd_chunks puts 4 neighbouring groups
+  // on one XCD; in the factor maps both colours of an instance follow one
+  // another in the same way.
   for (int l = 0; l < kMaxLev; ++l)
     for (int sh = 0; sh < kNumTileShapes; ++sh)
-      for (int d = 0; d < 2; ++d)
-        for (int p = 0; p < 2; ++p) {
-          auto &v = M.gmap[l][d][p][sh];
-          auto len = [&](const LineGrp &g) {
-            return d == 0 ? table[g.inst].lev[l].nx : table[g.inst].lev[l].ny;
-          };
-          for (auto *w : {&v, &M.sw[l][d][p][sh]}) {
-            std::stable_sort(w->begin(), w->end(),
-                             [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
-            if (d == 1) xcd_chunks(*w);
-          }
+      for (int d = 0; d < 2; ++d) {
+        auto len = [&](const LineGrp &g) {
+          return d == 0 ? table[g.inst].lev[l].nx : table[g.inst].lev[l].ny;
+        };
+        for (int p = 0; p < 3; ++p) {
+          auto &w = p < 2 ? M.sw[l][d][p][sh] : M.gfac[l][d][sh];
+          std::stable_sort(w.begin(), w.end(),
+                           [&](const LineGrp &a, const LineGrp &b) { return len(a) > len(b); });
+          if (d == 1) xcd_chunks(w);
         }
+      }
   for (int l = 0; l < kMaxLev; ++l) {
     xcd_blocks(M.cmap[l], 1);
     xcd_blocks(M.pmap[l], 1);
@@ -742,11 +747,10 @@ int Batch::rebuild() {
     sz(M.cmap[l].size(), sizeof(BlkMap)); sz(M.pmap[l].size(), sizeof(BlkMap));
     sz(M.rmap[l].size(), sizeof(BlkMap)); sz(M.qmap[l].size(), sizeof(BlkMap));
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      for (int d = 0; d < 2; ++d)
-        for (int p = 0; p < 2; ++p) {
-          sz(M.gmap[l][d][p][sh].size(), sizeof(LineGrp));
-          sz(M.sw[l][d][p][sh].size(), sizeof(LineGrp));
-        }
+      for (int d = 0; d < 2; ++d) {
+        sz(M.gfac[l][d][sh].size(), sizeof(LineGrp));
+        for (int p = 0; p < 2; ++p) sz(M.sw[l][d][p][sh].size(), sizeof(LineGrp));
+      }
       sz(M.gcx[l][sh].size(), sizeof(LineGrp)); sz(M.gcy[l][sh].size(), sizeof(LineGrp));
       sz(M.swx[l][sh].size(), sizeof(LineGrp)); sz(M.swy[l][sh].size(), sizeof(LineGrp));
     }
@@ -779,15 +783,16 @@ int Batch::rebuild() {
     dmn.qmap[l] = (const BlkMap *)put(M.qmap[l].data(), M.qmap[l].size(), sizeof(BlkMap));
     dmn.nqmap[l] = (int)M.qmap[l].size();
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      for (int d = 0; d < 2; ++d)
+      for (int d = 0; d < 2; ++d) {
+        auto &v = M.gfac[l][d][sh];
+        dmn.gfac[l][d][sh] = (const LineGrp *)put(v.data(), v.size(), sizeof(LineGrp));
+        dmn.ngfac[l][d][sh] = (int)v.size();
         for (int p = 0; p < 2; ++p) {
-          auto &v = M.gmap[l][d][p][sh];
-          dmn.gmap[l][d][p][sh] = (const LineGrp *)put(v.data(), v.size(), sizeof(LineGrp));
-          dmn.ngmap[l][d][p][sh] = (int)v.size();
           auto &w = M.sw[l][d][p][sh];
           dmn.sw[l][d][p][sh] = (const LineGrp *)put(w.data(), w.size(), sizeof(LineGrp));
           dmn.nsw[l][d][p][sh] = (int)w.size();
         }
+      }
       dmn.gcx[l][sh] = (const LineGrp *)put(M.gcx[l][sh].data(), M.gcx[l][sh].size(), sizeof(LineGrp));
       dmn.ngcx[l][sh] = (int)M.gcx[l][sh].size();
       dmn.gcy[l][sh] = (const LineGrp *)put(M.gcy[l][sh].data(), M.gcy[l][sh].size(), sizeof(LineGrp));
@@ -1000,12 +1005,12 @@ void Batch::setup_kernels(bool start64) {
                mixed ? opts.max_inner : opts.max_iters, opts.max_iters);
   tbegin(0); launch_assemble(stream, d_insts, dm.cmap[0], nb0); tend(0);
   // LU factors of the line matrices of level l: smoothing lines of both
-  // colours and the exact solve on the coarsest level
+  // colours (one launch per direction and tile shape) and the exact solve on
+  // the coarsest level
   auto factors = [&](int l, hipStream_t s) {
     for (int d = 0; d < 2; ++d)
-      for (int p = 0; p < 2; ++p)
-        for (int sh = 0; sh < kNumTileShapes; ++sh)
-          launch_line_factor(s, d_insts, dm.gmap[l][d][p][sh], dm.ngmap[l][d][p][sh], l, d, sh, h32);
+      for (int sh = 0; sh < kNumTileShapes; ++sh)
+        launch_line_factor(s, d_insts, dm.gfac[l][d][sh], dm.ngfac[l][d][sh], l, d, sh, h32);
     for (int sh = 0; sh < kNumTileShapes; ++sh) {
       launch_line_factor(s, d_insts, dm.gcx[l][sh], dm.ngcx[l][sh], l, 0, sh, h32);
       launch_line_factor(s, d_insts, dm.gcy[l][sh], dm.ngcy[l][sh], l, 1, sh, h32);
