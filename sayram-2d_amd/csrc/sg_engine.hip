@@ -370,9 +370,12 @@ static void apply_defaults(sg_opts &o) {
   if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 1;
-  // an x-line pre-sweep, then post-smoothing with x- and y-lines
-  if (o.smoother <= 0 || o.smoother > 15) o.smoother = 13;
-  if (o.coarse_smoother == 0) o.coarse_smoother = o.smoother;
+  // Default schedule: post-smoothing with x- and y-lines on the finest level,
+  // and on the coarser levels an x-line pre-sweep before it. An explicit
+  // smoother with coarse_smoother 0 applies to every level.
+  const bool sm_default = o.smoother <= 0 || o.smoother > 15;
+  if (sm_default) o.smoother = 12;
+  if (o.coarse_smoother == 0) o.coarse_smoother = sm_default ? 13 : o.smoother;
   if (o.fuse_cells == 0) o.fuse_cells = kFuseCells;
   if (o.streams == 0) o.streams = 1;
   if (o.group_cells == 0) o.group_cells = 1 << 21;

@@ -211,14 +211,16 @@ def test_two_groups_equal_one_group_bitwise(gpu_lib):
     assert [x[0] for x in out[5000][1]] == [4, 3, 5, 3, 4]
 
 
-@pytest.mark.parametrize("smoother", [12, 13, 15, 5])
+@pytest.mark.parametrize("smoother", [0, 12, 13, 15, 5])
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_smoother_schedules_stay_on_the_oracle(gpu_lib, smoother, precision):
-    """Post-smoothing alone (12), an x-line pre-sweep before it (13, the
-    default: the fused cycle then pre-smooths too, and only the even rows'
-    residual is restricted), pre-sweeps in both directions (15: every
-    residual formed) and x-lines only (5): different preconditioners, the same
-    solution to the solve tolerance, in every precision."""
+    """The default (0: post-smoothing alone on the finest level, an x-line
+    pre-sweep before it on the coarser ones), and on every level:
+    post-smoothing alone (12), an x-line pre-sweep before it (13: the fused
+    cycle then pre-smooths too, and only the even rows' residual is
+    restricted), pre-sweeps in both directions (15: every residual formed) and
+    x-lines only (5): different preconditioners, the same solution to the
+    solve tolerance, in every precision."""
     probs = [sc.seeded_case(nx, ny, seed=160 + k, scale=s) for k, (nx, ny, s) in
              enumerate([(70, 45, 0.5), (33, 130, 2.0), (120, 96, 1.0), (9, 300, 3.0)])]
     b = gpu_lib.Batch(gpu_lib.make_opts(precision=precision, smoother=smoother))
