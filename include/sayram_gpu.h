@@ -113,9 +113,11 @@ typedef struct {
                            0 selects 1024, the maximum (a level is kept in
                            LDS); -1 disables. Like line_tiles it changes the
                            rounding of those line solves. */
-  int streams;          /* 2: the tile shapes of one line sweep run on two HIP
-                           streams (the longest lines beside the rest), and the
-                           line factorisations beside the Galerkin chain; 1:
+  int streams;          /* 2: the tile shapes of one finest-level line sweep
+                           run on two HIP streams (the longest lines beside the
+                           rest; a coarser level's shapes share one launch in
+                           either mode), and the line factorisations beside
+                           the Galerkin chain; 1:
                            one stream. 0 selects 1: with the large instances in
                            a concurrent group of their own and a step captured
                            as one graph, the second stream no longer pays

@@ -157,6 +157,11 @@ struct Maps {
   std::vector<BlkMap> rmap[kMaxLev], qmap[kMaxLev];
   std::vector<LineGrp> sw[kMaxLev][2][2][kNumTileShapes], swx[kMaxLev][kNumTileShapes],
       swy[kMaxLev][kNumTileShapes];
+  // the 256-thread shapes of one colour's sweep of a 9-point level, in one
+  // map (rows: shapes 2, 1, 0; columns: 4, 0), and the first two shapes'
+  // group counts
+  std::vector<LineGrp> swm[kMaxLev][2][2];
+  int swm_na[kMaxLev][2][2], swm_nb[kMaxLev][2][2];
   std::vector<FuseMap> fmap;
 };
 
@@ -231,6 +236,8 @@ struct DevMaps {   // offsets into one device buffer
       *swy[kMaxLev][kNumTileShapes];
   int nsw[kMaxLev][2][2][kNumTileShapes], nswx[kMaxLev][kNumTileShapes],
       nswy[kMaxLev][kNumTileShapes];
+  const LineGrp *swm[kMaxLev][2][2];
+  int nswm[kMaxLev][2][2], swm_na[kMaxLev][2][2], swm_nb[kMaxLev][2][2];
   const FuseMap *fmap;
   int nfmap;
   int maxlev;
@@ -257,9 +264,10 @@ struct Batch {
   sg_opts opts{};
   size_t budget = 0, in_use = 0;
   hipStream_t stream = nullptr;
-  // Second stream (opts.streams == 2): the tile shapes of one sweep solve
-  // disjoint lines, so the shape with the longest lines runs on `side` beside
-  // the others on `stream`; `stream` waits for it before the next sweep.
+  // Second stream (opts.streams == 2): the tile shapes of one finest-level
+  // sweep solve disjoint lines, so the shape with the longest lines runs on
+  // `side` beside the others on `stream`; `stream` waits for it before the
+  // next sweep.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_lev[kMaxLev] = {};   // with `side`: level l's operator is built
@@ -657,7 +665,7 @@ void Batch::drop_graphs() {
 
 int Batch::rebuild() {
   drop_graphs();
-  Maps M;
+  Maps M{};
   order.clear();
   const bool f64 = opts.precision == SG_FP64;   // FP64 V-cycle
   // the lines the first post-smoothing sweep of level l reads
@@ -732,6 +740,21 @@ int Batch::rebuild() {
           if (d == 1) xcd_chunks(w);
         }
       }
+  // merged sweep maps of the 9-point levels (the shapes keep their order:
+  // each shape's groups longest first, column groups in XCD chunks, whose
+  // members stay on one XCD at any offset in the map)
+  for (int l = 1; l < kMaxLev; ++l)
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) {
+        static const int order[2][3] = {{2, 1, 0}, {4, 0, -1}};
+        auto &v = M.swm[l][d][p];
+        for (int k = 0; k < 3 && order[d][k] >= 0; ++k) {
+          const auto &w = M.sw[l][d][p][order[d][k]];
+          if (k == 0) M.swm_na[l][d][p] = (int)w.size();
+          if (k == 1) M.swm_nb[l][d][p] = (int)w.size();
+          v.insert(v.end(), w.begin(), w.end());
+        }
+      }
   for (int l = 0; l < kMaxLev; ++l) {
     xcd_blocks(M.cmap[l], 1);
     xcd_blocks(M.pmap[l], 1);
@@ -757,6 +780,8 @@ int Batch::rebuild() {
       sz(M.gcx[l][sh].size(), sizeof(LineGrp)); sz(M.gcy[l][sh].size(), sizeof(LineGrp));
       sz(M.swx[l][sh].size(), sizeof(LineGrp)); sz(M.swy[l][sh].size(), sizeof(LineGrp));
     }
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) sz(M.swm[l][d][p].size(), sizeof(LineGrp));
   }
   HIPCHK(hipStreamSynchronize(stream));
   if (bytes > d_maps_bytes) {
@@ -805,6 +830,14 @@ int Batch::rebuild() {
       dmn.swy[l][sh] = (const LineGrp *)put(M.swy[l][sh].data(), M.swy[l][sh].size(), sizeof(LineGrp));
       dmn.nswy[l][sh] = (int)M.swy[l][sh].size();
     }
+    for (int d = 0; d < 2; ++d)
+      for (int p = 0; p < 2; ++p) {
+        auto &v = M.swm[l][d][p];
+        dmn.swm[l][d][p] = (const LineGrp *)put(v.data(), v.size(), sizeof(LineGrp));
+        dmn.nswm[l][d][p] = (int)v.size();
+        dmn.swm_na[l][d][p] = M.swm_na[l][d][p];
+        dmn.swm_nb[l][d][p] = M.swm_nb[l][d][p];
+      }
   }
   dmn.maxlev = maxlev;
   if (bytes) HIPCHK(hipMemcpy(d_maps, host.data(), bytes, hipMemcpyHostToDevice));
@@ -894,12 +927,30 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      fan_out(m.nsw[l][dir][p], [&](hipStream_t s, int sh) {
-        tbegin(lid, s);   // one timed interval per kernel dispatch
-        launch_line<T>(s, d_insts, m.sw[l][dir][p][sh], m.nsw[l][dir][p][sh], l, dir, sh, zero,
-                       xid, bid);
-        tend(lid, s);
-      });
+      if (l >= 1) {
+        // a 9-point level, with one stream or two: the 1024-thread long-column
+        // groups, then every other shape of the colour in one launch (so that
+        // both stream modes run the same kernels and agree bit for bit)
+        if (m.nsw[l][dir][p][3]) {
+          tbegin(lid);
+          launch_line<T>(stream, d_insts, m.sw[l][dir][p][3], m.nsw[l][dir][p][3], l, dir, 3,
+                         zero, xid, bid);
+          tend(lid);
+        }
+        if (m.nswm[l][dir][p]) {
+          tbegin(lid);
+          launch_line_shapes<T>(stream, d_insts, m.swm[l][dir][p], m.swm_na[l][dir][p],
+                                m.swm_nb[l][dir][p], m.nswm[l][dir][p], l, dir, zero, xid, bid);
+          tend(lid);
+        }
+      } else {
+        fan_out(m.nsw[l][dir][p], [&](hipStream_t s, int sh) {
+          tbegin(lid, s);   // one timed interval per kernel dispatch
+          launch_line<T>(s, d_insts, m.sw[l][dir][p][sh], m.nsw[l][dir][p][sh], l, dir, sh, zero,
+                         xid, bid);
+          tend(lid, s);
+        });
+      }
       zero = false;
     }
   };

@@ -787,21 +787,28 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
 // holds is parked in x (no other line of this colour reads it). The backward
 // pass turns d into the solution, reading back only the parked tiles. ZERO:
 // the other lines are zero (first sweep of a cycle).
+// LDS of a line sweep of shape SH: the ring of KT tile buffers, the tile's
+// lower band (then upper factor) and inverse pivots, and the carry per line.
+template <typename T, int NB, int SH> constexpr int line_lds_elems() {
+  return (ring_tiles(SH, sizeof(T), NB) + 2) * tile_lines(SH) * (tile_steps(SH) + 4) +
+         tile_lines(SH);
+}
+
+// The sweep of one line group gr by one workgroup of tile_threads(SH)
+// threads, in the LDS lds (line_lds_elems<T, NB, SH>() values).
 template <typename T, int NB, int DIR, bool ZERO, int SH>
-__global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__restrict__ insts,
-                                                           const LineGrp *__restrict__ map, int l,
-                                                           int xid, int bid) {
+__device__ __forceinline__ void line_sweep(const InstDev *__restrict__ insts, const LineGrp gr,
+                                           int l, int xid, int bid, T *lds) {
   constexpr int S = tile_steps(SH), L = tile_lines(SH), NT = tile_threads(SH);
   constexpr int W = NT / L;          // threads per line in the solve, 4 steps each
   constexpr int Q = S * L / NT;      // elements per thread in the loads
   constexpr int KT = ring_tiles(SH, sizeof(T), NB);   // tiles of d kept in LDS
   constexpr int TB = L * (S + 4);                  // one tile buffer
   static_assert(W * 4 == S && W <= 64, "a line's threads cover its tile in one wave");
-  __shared__ T s_rb[KT * TB];   // r, then d, then x of a tile, in a ring of KT
-  __shared__ T s_a[TB];
-  __shared__ T s_i[TB];
-  __shared__ T s_c[L];
-  const LineGrp gr = map[blockIdx.x];
+  T *const s_rb = lds;             // r, then d, then x of a tile, in a ring of KT
+  T *const s_a = s_rb + KT * TB;
+  T *const s_i = s_a + TB;
+  T *const s_c = s_i + TB;         // L values
   const InstDev &I = insts[gr.inst];
   if (rsc(I)->status != ST_ACTIVE) return;
   const Lev &Lv = I.lev[l];
@@ -1008,6 +1015,39 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
     }
     lds_barrier();
   }
+}
+
+template <typename T, int NB, int DIR, bool ZERO, int SH>
+__global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__restrict__ insts,
+                                                           const LineGrp *__restrict__ map, int l,
+                                                           int xid, int bid) {
+  __shared__ T lds[line_lds_elems<T, NB, SH>()];
+  line_sweep<T, NB, DIR, ZERO, SH>(insts, map[blockIdx.x], l, xid, bid, lds);
+}
+
+// The tile shapes of one colour's sweep in one launch (256-thread shapes):
+// map holds the groups of shape SA (the first na entries), then of SB (the
+// next nb), then of SC (the rest; SC < 0: none). Each workgroup runs the
+// sweep of its shape in the largest of their LDS layouts. One launch instead
+// of one per shape: the shapes' workgroups overlap instead of each shape
+// filling and draining the GPU in turn.
+template <typename T, int NB, int DIR, bool ZERO, int SA, int SB, int SC>
+__global__ void __launch_bounds__(256) k_line_shapes(const InstDev *__restrict__ insts,
+                                                     const LineGrp *__restrict__ map, int na,
+                                                     int nb, int l, int xid, int bid) {
+  static_assert(tile_threads(SA) == 256 && tile_threads(SB) == 256 &&
+                    (SC < 0 || tile_threads(SC) == 256),
+                "one workgroup size");
+  constexpr int SC_ = SC < 0 ? SB : SC;
+  constexpr int EA = line_lds_elems<T, NB, SA>(), EB = line_lds_elems<T, NB, SB>();
+  constexpr int EC = SC < 0 ? 0 : line_lds_elems<T, NB, SC_>();
+  constexpr int E = EA > EB ? (EA > EC ? EA : EC) : (EB > EC ? EB : EC);
+  __shared__ T lds[E];
+  const int w = blockIdx.x;
+  const LineGrp gr = map[w];
+  if (w < na) line_sweep<T, NB, DIR, ZERO, SA>(insts, gr, l, xid, bid, lds);
+  else if (SC < 0 || w < na + nb) line_sweep<T, NB, DIR, ZERO, SB>(insts, gr, l, xid, bid, lds);
+  else line_sweep<T, NB, DIR, ZERO, SC_>(insts, gr, l, xid, bid, lds);
 }
 
 // (A x)(i, j) at slot K. Every operand is loaded before the sum, so that a
@@ -2314,6 +2354,23 @@ template void launch_line<double>(hipStream_t, const InstDev *, const LineGrp *,
                                   int, bool, int, int);
 template void launch_line<float>(hipStream_t, const InstDev *, const LineGrp *, int, int, int,
                                  int, bool, int, int);
+
+template <typename T>
+void launch_line_shapes(hipStream_t s, const InstDev *d, const LineGrp *m, int na, int nb,
+                        int ngroups, int l, int dir, bool zero, int xid, int bid) {
+  if (!ngroups) return;
+  if (dir == 0) {
+    if (zero) k_line_shapes<T, 9, 0, true, 2, 1, 0><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+    else k_line_shapes<T, 9, 0, false, 2, 1, 0><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+  } else {
+    if (zero) k_line_shapes<T, 9, 1, true, 4, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+    else k_line_shapes<T, 9, 1, false, 4, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+  }
+}
+template void launch_line_shapes<double>(hipStream_t, const InstDev *, const LineGrp *, int, int,
+                                         int, int, int, bool, int, int);
+template void launch_line_shapes<float>(hipStream_t, const InstDev *, const LineGrp *, int, int,
+                                        int, int, int, bool, int, int);
 
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,

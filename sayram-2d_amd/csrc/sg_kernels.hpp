@@ -15,6 +15,12 @@ void launch_line_factor(hipStream_t s, const InstDev *d, const LineGrp *m, int n
 template <typename T>
 void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups, int l, int dir,
                  int shape, bool zero, int xid, int bid);
+// one colour of a 9-point sweep over several tile shapes in one launch: m
+// holds ngroups line groups, rows in shapes 2, 1, 0 and columns in shapes
+// 4, 0 (na groups of the first shape, then nb of the second, then the rest)
+template <typename T>
+void launch_line_shapes(hipStream_t s, const InstDev *d, const LineGrp *m, int na, int nb,
+                        int ngroups, int l, int dir, bool zero, int xid, int bid);
 template <typename T>
 void launch_residual(hipStream_t s, const InstDev *d, const BlkMap *m, int nb, int l, int xid,
                      int bid);
