@@ -787,6 +787,7 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line_factor(const InstDev 
 // holds is parked in x (no other line of this colour reads it). The backward
 // pass turns d into the solution, reading back only the parked tiles. ZERO:
 // the other lines are zero (first sweep of a cycle).
+//
 // LDS of a line sweep of shape SH: the ring of KT tile buffers, the tile's
 // lower band (then upper factor) and inverse pivots, and the carry per line.
 template <typename T, int NB, int SH> constexpr int line_lds_elems() {
@@ -795,7 +796,8 @@ template <typename T, int NB, int SH> constexpr int line_lds_elems() {
 }
 
 // The sweep of one line group gr by one workgroup of tile_threads(SH)
-// threads, in the LDS lds (line_lds_elems<T, NB, SH>() values).
+// threads, in the LDS lds (line_lds_elems<T, NB, SH>() values); k_line and
+// k_line_shapes below are its kernels.
 template <typename T, int NB, int DIR, bool ZERO, int SH>
 __device__ __forceinline__ void line_sweep(const InstDev *__restrict__ insts, const LineGrp gr,
                                            int l, int xid, int bid, T *lds) {
