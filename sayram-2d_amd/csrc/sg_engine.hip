@@ -158,10 +158,13 @@ struct Maps {
   std::vector<LineGrp> sw[kMaxLev][2][2][kNumTileShapes], swx[kMaxLev][kNumTileShapes],
       swy[kMaxLev][kNumTileShapes];
   // the 256-thread shapes of one colour's sweep of a 9-point level, in one
-  // map (rows: shapes 2, 1, 0; columns: 4, 0), and the first two shapes'
-  // group counts
+  // map (rows: shapes 2, 1, 0; columns: 4, 0, or 1, 0 with FP64 V-cycles),
+  // the first two shapes' group counts, and whether the map and the
+  // 1024-thread shape 3 hold every group of the colour (else the shapes are
+  // launched one by one)
   std::vector<LineGrp> swm[kMaxLev][2][2];
   int swm_na[kMaxLev][2][2], swm_nb[kMaxLev][2][2];
+  bool swm_all[kMaxLev][2][2];
   std::vector<FuseMap> fmap;
 };
 
@@ -238,6 +241,7 @@ struct DevMaps {   // offsets into one device buffer
       nswy[kMaxLev][kNumTileShapes];
   const LineGrp *swm[kMaxLev][2][2];
   int nswm[kMaxLev][2][2], swm_na[kMaxLev][2][2], swm_nb[kMaxLev][2][2];
+  bool swm_all[kMaxLev][2][2];
   const FuseMap *fmap;
   int nfmap;
   int maxlev;
@@ -743,17 +747,22 @@ int Batch::rebuild() {
   // merged sweep maps of the 9-point levels (the shapes keep their order:
   // each shape's groups longest first, column groups in XCD chunks, whose
   // members stay on one XCD at any offset in the map)
+  const bool fp64_cycle = opts.precision == SG_FP64;
   for (int l = 1; l < kMaxLev; ++l)
     for (int d = 0; d < 2; ++d)
       for (int p = 0; p < 2; ++p) {
-        static const int order[2][3] = {{2, 1, 0}, {4, 0, -1}};
+        // the shapes launch_line_shapes takes, in its order
+        const int order[2][3] = {{2, 1, 0}, {fp64_cycle ? 1 : 4, 0, -1}};
         auto &v = M.swm[l][d][p];
+        size_t all = 0;
+        for (int sh = 0; sh < kNumTileShapes; ++sh) all += M.sw[l][d][p][sh].size();
         for (int k = 0; k < 3 && order[d][k] >= 0; ++k) {
           const auto &w = M.sw[l][d][p][order[d][k]];
           if (k == 0) M.swm_na[l][d][p] = (int)w.size();
           if (k == 1) M.swm_nb[l][d][p] = (int)w.size();
           v.insert(v.end(), w.begin(), w.end());
         }
+        M.swm_all[l][d][p] = v.size() + M.sw[l][d][p][3].size() == all;
       }
   for (int l = 0; l < kMaxLev; ++l) {
     xcd_blocks(M.cmap[l], 1);
@@ -837,6 +846,7 @@ int Batch::rebuild() {
         dmn.nswm[l][d][p] = (int)v.size();
         dmn.swm_na[l][d][p] = M.swm_na[l][d][p];
         dmn.swm_nb[l][d][p] = M.swm_nb[l][d][p];
+        dmn.swm_all[l][d][p] = M.swm_all[l][d][p];
       }
   }
   dmn.maxlev = maxlev;
@@ -927,7 +937,7 @@ template <typename T> void Batch::vcycle(int l, int xid, int bid) {
   bool zero = true;
   auto sweep = [&](int dir) {
     for (int p = 0; p < 2; ++p) {
-      if (l >= 1) {
+      if (l >= 1 && m.swm_all[l][dir][p]) {
         // a 9-point level, with one stream or two: the 1024-thread long-column
         // groups, then every other shape of the colour in one launch (so that
         // both stream modes run the same kernels and agree bit for bit)

@@ -2365,8 +2365,11 @@ void launch_line_shapes(hipStream_t s, const InstDev *d, const LineGrp *m, int n
     if (zero) k_line_shapes<T, 9, 0, true, 2, 1, 0><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
     else k_line_shapes<T, 9, 0, false, 2, 1, 0><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
   } else {
-    if (zero) k_line_shapes<T, 9, 1, true, 4, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
-    else k_line_shapes<T, 9, 1, false, 4, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+    // columns: shapes 4, 0 (FP32 V-cycles) or 1, 0 (FP64: tile_shape gives
+    // their columns 16 x 64 tiles)
+    constexpr int SA = sizeof(T) == 4 ? 4 : 1;
+    if (zero) k_line_shapes<T, 9, 1, true, SA, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
+    else k_line_shapes<T, 9, 1, false, SA, 0, -1><<<ngroups, 256, 0, s>>>(d, m, na, nb, l, xid, bid);
   }
 }
 template void launch_line_shapes<double>(hipStream_t, const InstDev *, const LineGrp *, int, int,

@@ -17,7 +17,8 @@ void launch_line(hipStream_t s, const InstDev *d, const LineGrp *m, int ngroups,
                  int shape, bool zero, int xid, int bid);
 // one colour of a 9-point sweep over several tile shapes in one launch: m
 // holds ngroups line groups, rows in shapes 2, 1, 0 and columns in shapes
-// 4, 0 (na groups of the first shape, then nb of the second, then the rest)
+// 4, 0 with T = float or 1, 0 with T = double (na groups of the first shape,
+// then nb of the second, then the rest)
 template <typename T>
 void launch_line_shapes(hipStream_t s, const InstDev *d, const LineGrp *m, int na, int nb,
                         int ngroups, int l, int dir, bool zero, int xid, int bid);

@@ -9,6 +9,10 @@ while others run are admitted in arrival order as soon as the HBM budget has
 room (sg_instance_bytes tells the cost before anything is allocated); a request
 that reaches its step count is retired at the end of that tick (its f is
 copied to the host and its slab freed), which frees budget for the next ones.
+A tick is as many steps as the resident request nearest to its end still
+needs, so that it ends when that request does; while arrivals are still due
+and the budget could take them, a tick is one step, so that each one is
+admitted at the next step.
 
 Because every reduction is per instance and in a fixed order, a request's
 result does not depend on what else is resident: it is bit-identical to the
@@ -36,6 +40,7 @@ class Request:
     steps: int
     admitted: float = math.nan
     finished: float = math.nan
+    done_steps: int = 0     # steps taken so far (after the last tick)
     result: np.ndarray | None = None
     stats: dict = field(default_factory=dict)
     error: str | None = None   # why the request was rejected or stopped
@@ -117,19 +122,26 @@ class ServeLoop:
                 if i < len(pending):     # idle until the next arrival
                     time.sleep(max(0.0, pending[i].arrival - now()))
                 continue
-            # one tick: every resident request advances one step, none past its target
+            # one tick: every resident request advances, none past its target;
+            # k steps, k the fewest any resident still needs (one step while
+            # arrivals are due and nothing waits for budget: a new arrival
+            # is then admitted at the next step)
             nslot = max(resident) + 1
             targets = np.full(nslot, -1, dtype=np.int64)
             for s, r in resident.items():
                 targets[s] = r.steps
+            k = min(r.steps - r.done_steps for r in resident.values())
+            if i < len(pending) and not queue:
+                k = 1
             tb = time.perf_counter()
-            rc = self.batch.step_until(targets, 1)
+            rc = self.batch.step_until(targets, max(1, k))
             self.busy_s += time.perf_counter() - tb
             if rc not in (sg.SG_OK, sg.SG_ERR_NOCONV):
                 raise sg.SayramError(rc, sg.lib().sg_last_error().decode())
             self.ticks += 1
             for s in list(resident):
                 st = self.batch.stats(s)
+                resident[s].done_steps = st["steps"]
                 failed = st["status"] == sg.SG_ERR_NOCONV
                 if st["steps"] >= resident[s].steps or failed:
                     r = resident.pop(s)

@@ -286,3 +286,30 @@ def test_batch_resume_one_slot(gpu_lib):
         b.set_f(99, np.zeros((3, 3)), 0)
     b.close()
     ref.close()
+
+
+def test_fp64_vcycle_as_strong_as_fp32(gpu_lib):
+    """The FP64 and FP32 V-cycles run the same schedule on the same hierarchy,
+    one in higher precision: per instance and step the FP64 solve needs at
+    most one iteration more than the PC32 one (with every coarse line swept
+    they agree to about that). A sweep that skipped some line groups of a
+    level, in one precision only, would still converge, only in more
+    iterations: this is what shows it. The shapes cover the 9-point levels'
+    rows and columns in every tile shape both precisions use (lines of at
+    most 16 cells, medium and long)."""
+    sg = gpu_lib
+    shapes = [(200, 150), (300, 90), (96, 400), (128, 128), (520, 40), (36, 610), (900, 700)]
+    probs = [sc.seeded_case(nx, ny, seed=170 + k, scale=s)
+             for k, ((nx, ny), s) in enumerate(zip(shapes, (0.5, 2.0, 4.0, 1.0, 6.0, 0.3, 1.5)))]
+    its = {}
+    for prec in (sg.SG_PC32, sg.SG_FP64):
+        b = sg.Batch(sg.make_opts(precision=prec))
+        slots = [b.add(p) for p in probs]
+        its[prec] = []
+        for _ in range(4):
+            b.step(1)
+            its[prec].append([b.stats(s)["last_iters"] for s in slots])
+        b.close()
+    for step, (a, c) in enumerate(zip(its[sg.SG_PC32], its[sg.SG_FP64])):
+        for k, (i32, i64) in enumerate(zip(a, c)):
+            assert i64 <= i32 + 1, f"{shapes[k]} step {step + 1}: FP64 {i64} vs PC32 {i32} iterations"
