@@ -1032,9 +1032,12 @@ __global__ void __launch_bounds__(tile_threads(SH)) k_line(const InstDev *__rest
 // next nb), then of SC (the rest; SC < 0: none). Each workgroup runs the
 // sweep of its shape in the largest of their LDS layouts. One launch instead
 // of one per shape: the shapes' workgroups overlap instead of each shape
-// filling and draining the GPU in turn.
+// filling and draining the GPU in turn. The FP32 row kernel is held to 4
+// waves per SIMD (128 VGPRs, no spills; without the bound its three shape
+// bodies take 140 and it runs 3, as the shapes alone do not).
 template <typename T, int NB, int DIR, bool ZERO, int SA, int SB, int SC>
-__global__ void __launch_bounds__(256) k_line_shapes(const InstDev *__restrict__ insts,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIR == 0 && sizeof(T) == 4 ? 4 : 1)))
+k_line_shapes(const InstDev *__restrict__ insts,
                                                      const LineGrp *__restrict__ map, int na,
                                                      int nb, int l, int xid, int bid) {
   static_assert(tile_threads(SA) == 256 && tile_threads(SB) == 256 &&
