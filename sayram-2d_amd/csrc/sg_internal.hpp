@@ -28,7 +28,7 @@ constexpr int kNumPartials = 5;
 // may address it with 32-bit byte offsets from its base.
 constexpr long long kMaxCells = 1LL << 28;
 constexpr int kBlock = 256;     // cells per workgroup in cell-parallel kernels
-constexpr int kLightBlocks = 2; // map entries per workgroup of the lightest vector kernels
+constexpr int kLightBlocks = 4; // map entries per workgroup of the lightest vector kernels
 constexpr int kEnt = 4;         // map entries per workgroup of the multi-entry cell kernels
 // Tile shapes of the line solves: L lines x S steps staged through LDS by a
 // workgroup of NT threads, each thread owning 4 consecutive steps of a line in

@@ -1713,11 +1713,12 @@ __device__ __forceinline__ T precond_vec(const InstDev &I, int id, int pcid, siz
 // FP32 by every later iteration, with rho0 = (rh, r0) = ||r0|| to FP32
 // rounding.
 //
-// A workgroup takes two map entries (kLightBlocks): the chain of dependent
+// A workgroup takes kLightBlocks (4) map entries: the chain of dependent
 // reads in front of an entry's vector traffic (map entry, instance, its
 // scalars) is latency this light kernel cannot hide within one 256-cell
-// block, and the second entry's chain is issued while the first one's vectors
-// move (the map, the instance table and the scalars are read-only here).
+// block, and the next entries' chains are issued while the first one's
+// vectors move (the map, the instance table and the scalars are read-only
+// here).
 template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_p(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map, int nmap) {
@@ -1782,7 +1783,7 @@ template <typename T, bool PC>
 __global__ void __launch_bounds__(kBlock) k_bicg_s(const InstDev *__restrict__ insts,
                                                   const BlkMap *__restrict__ map, int nmap) {
 #pragma unroll
-  for (int e = 0; e < kLightBlocks; ++e) {   // two map entries, as k_bicg_p
+  for (int e = 0; e < kLightBlocks; ++e) {   // kLightBlocks map entries, as k_bicg_p
     const int bi = kLightBlocks * blockIdx.x + e;
     if (bi >= nmap) break;
     const BlkMap m = map[bi];
