@@ -82,3 +82,32 @@ def test_stretched_grids_batched_against_oracle(gpu_lib, precision):
             assert e <= TOL, f"instance {k} step {step + 1}: {e:.3e}"
             assert b.f(sl).min() >= 0.0
     b.close()
+
+
+def test_every_coarse_tile_shape_in_one_batch_against_oracle(gpu_lib):
+    """The line groups of a coarse level's colour share one launch across
+    tile shapes, and which shapes occur depends on line length and V-cycle
+    precision. These grids give the first coarse level (more than 1024
+    cells, so swept by the per-level launches) rows of 200 and 250 (4 x 256
+    tiles), 20 and 60 (16 x 64) and 15 cells (64 x 16), and columns of 12
+    cells (64 x 16), of 20, 125 and 300 cells (32 x 32 tiles in FP32, 16 x 64
+    in FP64) and of 1000 cells (1024-thread 16 x 256 tiles in FP32). Batched
+    together, in every precision, every instance stays on the oracle every
+    step."""
+    sg = gpu_lib
+    shapes = [(400, 40), (40, 600), (30, 2000), (120, 250), (500, 24)]
+    probs = [sc.seeded_case(nx, ny, seed=61 + k, scale=s)
+             for k, ((nx, ny), s) in enumerate(zip(shapes, (2.0, 0.5, 1.0, 4.0, 1.5)))]
+    refs = [[p.f0.copy()] for p in probs]
+    for k, p in enumerate(probs):
+        for _ in range(3):
+            refs[k].append(orc.step(orc.OracleProblem(p), refs[k][-1]))
+    for precision in (0, 1, 2):
+        b = sg.Batch(sg.make_opts(precision=precision))
+        slots = [b.add(p) for p in probs]
+        for step in range(3):
+            assert b.step_status(1) == 0
+            for k, s in enumerate(slots):
+                e = rel_l2(b.f(s), refs[k][step + 1])
+                assert e <= TOL, f"precision {precision} {shapes[k]} step {step + 1}: {e:.3e}"
+        b.close()
